@@ -1,0 +1,240 @@
+"""feyntrop_amd -- MI355X tropical Monte Carlo sampler for Feynman integrals.
+
+Python binding of the C ABI in include/feyntrop_amd.h (libfeyntrop_amd.so,
+built in-tree by feyntrop_amd/csrc/Makefile).  The library is the only
+compute path: if it is missing this module raises on import of the engine.
+
+Reference operator interface mirrored (michibo/feyntrop):
+  Engine.run_json(text)      -- the ./feyntrop stdin/stdout JSON contract
+                                 (srcs/feyntrop.cpp:20-281)
+  Engine.prepare(problem)    -- graph + checks + regime + J-table
+                                 (feyntrop.cpp:45-239)
+  Engine.integrate(p, N, s)  -- feynman_integral_estimate
+                                 (feynman_integral.hpp:30-373) -> raw moments
+  Problem.finalize(moments)  -- stats avg/acc (stats.hpp:86-104)
+"""
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfeyntrop_amd.so")
+BIN_PATH = os.path.join(_HERE, "bin", "feyntrop")
+KCACHE = os.path.join(_HERE, "kcache")
+
+FTR_MAX_K = 16
+
+
+class ftr_input(C.Structure):
+    _fields_ = [("E", C.c_int), ("V", C.c_int), ("edges", C.POINTER(C.c_int32)),
+                ("nu", C.POINTER(C.c_double)), ("P", C.POINTER(C.c_double)),
+                ("m2", C.POINTER(C.c_double)), ("D", C.c_double), ("K", C.c_int),
+                ("lambda_", C.c_double)]
+
+
+class ftr_info(C.Structure):
+    _fields_ = [("E", C.c_int), ("V", C.c_int), ("L", C.c_int), ("K", C.c_int),
+                ("W", C.c_double), ("IGtr", C.c_double),
+                ("euclidean", C.c_int), ("pseudo_euclidean", C.c_int), ("generic", C.c_int),
+                ("gp_property", C.c_int), ("lambda_eff", C.c_double), ("path", C.c_int),
+                ("seconds_preprocessing", C.c_double)]
+
+
+class ftr_moments(C.Structure):
+    _fields_ = [("K", C.c_int),
+                ("n", (C.c_double * 2) * FTR_MAX_K),
+                ("s1", (C.c_double * 2) * FTR_MAX_K),
+                ("s2", (C.c_double * 2) * FTR_MAX_K),
+                ("dropped", C.c_uint64), ("kernel_ms", C.c_double), ("seconds", C.c_double),
+                ("lanes", C.c_uint64), ("samples_per_lane", C.c_uint64)]
+
+    def arrays(self):
+        K = self.K
+        n = np.array([[self.n[k][q] for q in range(2)] for k in range(K)])
+        s1 = np.array([[self.s1[k][q] for q in range(2)] for k in range(K)])
+        s2 = np.array([[self.s2[k][q] for q in range(2)] for k in range(K)])
+        return n, s1, s2
+
+
+class ftr_opts(C.Structure):
+    _fields_ = [("block", C.c_int), ("table_mode", C.c_int), ("lanes", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfeyntrop_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"feyntrop_amd: {LIB_PATH} not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.ftr_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.ftr_destroy.argtypes = [vp]
+    L.ftr_last_error.restype = C.c_char_p
+    L.ftr_last_error.argtypes = [vp]
+    L.ftr_prepare.argtypes = [vp, C.POINTER(ftr_input), C.POINTER(vp)]
+    L.ftr_problem_free.argtypes = [vp]
+    L.ftr_problem_info.argtypes = [vp, C.POINTER(ftr_info)]
+    L.ftr_jtable.argtypes = [vp, vp, vp, vp, vp]
+    L.ftr_integrate.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts), C.POINTER(ftr_moments)]
+    L.ftr_finalize.argtypes = [vp, C.POINTER(ftr_moments), vp]
+    L.ftr_sample.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp]
+    L.ftr_replay.argtypes = [vp, vp, C.c_uint64, vp, vp, vp, vp]
+    L.ftr_run_json.argtypes = [vp, C.c_char_p, C.POINTER(vp), C.POINTER(vp)]
+    L.ftr_free_string.argtypes = [vp]
+    L.ftr_integrate_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64), C.POINTER(ftr_moments)]
+    L.ftr_set_cache_dir.argtypes = [C.c_char_p]
+    L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
+    _lib = L
+    return L
+
+
+class _Input:
+    """Keeps the numpy buffers of an ftr_input alive."""
+
+    def __init__(self, prob):
+        g = prob["graph"]
+        self.edges = np.array([[int(e[0][0]), int(e[0][1])] for e in g], dtype=np.int32).reshape(-1)
+        self.nu = np.array([float(e[1]) for e in g], dtype=np.float64)
+        self.P = np.ascontiguousarray(np.array(prob["scalarproducts"], dtype=np.float64).reshape(-1))
+        self.m2 = np.array([float(m) for m in prob["masses_sqr"]], dtype=np.float64)
+        V = int(round(len(self.P) ** 0.5))
+        self.s = ftr_input(len(self.nu), V,
+                           self.edges.ctypes.data_as(C.POINTER(C.c_int32)),
+                           self.nu.ctypes.data_as(C.POINTER(C.c_double)),
+                           self.P.ctypes.data_as(C.POINTER(C.c_double)),
+                           self.m2.ctypes.data_as(C.POINTER(C.c_double)),
+                           float(prob["dimension"]), int(prob["num_eps_terms"]), float(prob["lambda"]))
+
+
+class FeyntropError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def precompile(prob):
+    """Specialise and cache the kernels of a problem (no GPU needed)."""
+    inp = _Input(prob)
+    err = C.create_string_buffer(8192)
+    rc = lib().ftr_precompile(C.byref(inp.s), err, 8192)
+    if rc:
+        raise FeyntropError(rc, err.value.decode(errors="replace"))
+
+
+class Problem:
+    def __init__(self, engine, handle, prob):
+        self.engine = engine
+        self.h = handle
+        self.prob = prob
+        info = ftr_info()
+        lib().ftr_problem_info(self.h, C.byref(info))
+        self.info = info
+        self.E, self.V, self.K = info.E, info.V, info.K
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ftr_problem_free(self.h)
+            self.h = None
+
+    def jtable(self):
+        n = 1 << self.E
+        Jr, r = np.zeros(n), np.zeros(n)
+        mm, L = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+        lib().ftr_jtable(self.h, Jr.ctypes.data, r.ctypes.data, mm.ctypes.data, L.ctypes.data)
+        return Jr, r, mm, L
+
+    def finalize(self, m):
+        res = np.zeros(self.K * 4)
+        lib().ftr_finalize(self.h, C.byref(m), res.ctypes.data)
+        return res.reshape(self.K, 2, 2)
+
+
+class Engine:
+    """One HIP device context (stream, kernel modules, scratch)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().ftr_create(device, C.byref(h))
+        if rc:
+            raise FeyntropError(rc, "ftr_create failed (no usable HIP device?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ftr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _err(self, rc):
+        return FeyntropError(rc, lib().ftr_last_error(self.h).decode(errors="replace"))
+
+    def prepare(self, prob):
+        inp = _Input(prob)
+        out = C.c_void_p()
+        rc = lib().ftr_prepare(self.h, C.byref(inp.s), C.byref(out))
+        if rc:
+            raise self._err(rc)
+        return Problem(self, out, prob)
+
+    def integrate(self, p, N, seed=0, lanes=0):
+        m = ftr_moments()
+        opts = ftr_opts(0, -1, int(lanes))
+        rc = lib().ftr_integrate(self.h, p.h, int(N), int(seed), C.byref(opts), C.byref(m))
+        if rc:
+            raise self._err(rc)
+        return m
+
+    def integrate_batch(self, probs, Ns, seeds):
+        n = len(probs)
+        hs = (C.c_void_p * n)(*[p.h for p in probs])
+        Na = (C.c_uint64 * n)(*[int(x) for x in Ns])
+        Sa = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        out = (ftr_moments * n)()
+        rc = lib().ftr_integrate_batch(self.h, n, hs, Na, Sa, out)
+        if rc:
+            raise self._err(rc)
+        return list(out)
+
+    def sample(self, p, first, n, seed, X_ptr, px_ptr):
+        """Device pointers (e.g. torch tensor .data_ptr()) of n*E and n*2 doubles."""
+        rc = lib().ftr_sample(self.h, p.h, int(first), int(n), int(seed), C.c_void_p(X_ptr), C.c_void_p(px_ptr))
+        if rc:
+            raise self._err(rc)
+
+    def replay(self, p, n, X_ptr, px_ptr, out_ptr, drop_ptr):
+        rc = lib().ftr_replay(self.h, p.h, int(n), C.c_void_p(X_ptr), C.c_void_p(px_ptr),
+                              C.c_void_p(out_ptr), C.c_void_p(drop_ptr))
+        if rc:
+            raise self._err(rc)
+
+    def run_json(self, text):
+        out, log = C.c_void_p(), C.c_void_p()
+        rc = lib().ftr_run_json(self.h, text.encode(), C.byref(out), C.byref(log))
+        o = C.string_at(out.value).decode() if out.value else None
+        lg = C.string_at(log.value).decode() if log.value else ""
+        lib().ftr_free_string(out)
+        lib().ftr_free_string(log)
+        return rc, o, lg
+
+
+def integral(engine, prob):
+    """The reference's result list [[(re, dre), (im, dim)], ...] for a problem dict."""
+    p = engine.prepare(prob)
+    m = engine.integrate(p, prob["N"], prob["seed"])
+    r = p.finalize(m)
+    return [[(r[k, 0, 0], r[k, 0, 1]), (r[k, 1, 0], r[k, 1, 1])] for k in range(p.K)], p.info.IGtr
+
+
+def to_json(prob):
+    return json.dumps(prob)

@@ -1,0 +1,625 @@
+/*
+ * runtime.cpp -- C ABI of the MI355X engine: device context, per-graph kernel
+ * specialisation (hipRTC, cached code objects), launches and moments.
+ *
+ * Hot path: one kernel per problem (ftrk_int).  Each lane owns a contiguous
+ * run of S samples of the single xoshiro256+ stream; the host supplies the
+ * stream state at every block's first sample (chained jump polynomials) and
+ * the in-block jump polynomials, so the samples are exactly those of the
+ * reference run with one OpenMP thread (feynman_integral.hpp:85-93, :148-164
+ * with max_threads = 1).  Blocks write partial moments; the host sums them in
+ * block order (deterministic).
+ */
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../../include/feyntrop_amd.h"
+#include "../kernels/ftr_abi.hpp"
+#include "ftr_host.hpp"
+
+extern const char* ftr_embedded_device_source;
+
+using ftrh::Error;
+
+struct Variant {
+  int mode = 0, tmode = 0, block = 256, K = 1;
+};
+
+struct Module {
+  hipModule_t mod = nullptr;
+  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr;
+};
+
+struct ftr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop;
+  std::string err;
+  std::map<std::string, Module> modules;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  /* growable device scratch: block states, lane polys, partials */
+  void* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  std::vector<double> h_partials;
+};
+
+struct ftr_problem {
+  ftrh::Prep prep;
+  ftr_ctx* ctx = nullptr;
+  Variant var;
+  size_t lds = 0;
+  Module m;
+  ftr::DevParams hprm;
+  ftr::DevParams* d_prm = nullptr;
+  double *d_w = nullptr, *d_jr = nullptr, *d_ir = nullptr;
+  uint8_t* d_mml = nullptr;
+};
+
+/* ------------------------------------------------------------ helpers -- */
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) throw Error(FTR_ERR_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static thread_local std::string g_host_err; /* errors of ctx-less calls */
+static std::string g_cache_dir;
+static std::mutex g_cache_mu;
+
+static std::string default_cache_dir() {
+  if (const char* e = getenv("FEYNTROP_AMD_KCACHE")) return e;
+  Dl_info info;
+  if (dladdr((void*)&default_cache_dir, &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    const size_t s = p.rfind('/');
+    std::string dir = (s == std::string::npos) ? "." : p.substr(0, s);
+    return dir + "/../kcache";
+  }
+  return "kcache";
+}
+
+static std::string cache_dir() {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  if (g_cache_dir.empty()) g_cache_dir = default_cache_dir();
+  return g_cache_dir;
+}
+
+static uint64_t fnv1a(const std::string& s) {
+  uint64_t h = 1469598103934665603ULL;
+  for (unsigned char c : s) {
+    h ^= c;
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+/* Kernel variant for a problem: path -> MODE, table size -> placement. */
+static Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
+  Variant v;
+  v.K = pr.K;
+  v.mode = pr.path;
+  const int E = pr.g.E;
+  const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
+  const size_t cap0 = std::min<size_t>(lds_limit, 64 * 1024);
+  if (full <= cap0) v.tmode = 0;
+  else if (wonly <= lds_limit) v.tmode = 1;
+  else v.tmode = 2;
+  v.block = 256;
+  return v;
+}
+
+static size_t variant_lds(const Variant& v, int E) {
+  size_t t = v.tmode == 0 ? ((size_t)25 << E) + 64 : (v.tmode == 1 ? ((size_t)8 << E) : 0);
+  t = (t + 15) & ~(size_t)15;
+  return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
+}
+
+static std::string spec_source(const ftrh::Prep& pr, const Variant& v) {
+  const ftrh::Graph& g = pr.g;
+  std::ostringstream s;
+  s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size()
+    << ";\n  static constexpr int hi[" << g.E << "] = {";
+  for (int e = 0; e < g.E; e++) s << (e ? "," : "") << g.hi[e];
+  s << "};\n  static constexpr int lo[" << g.E << "] = {";
+  for (int e = 0; e < g.E; e++) s << (e ? "," : "") << g.lo[e];
+  s << "};\n  static constexpr int ext[" << std::max<size_t>(1, pr.ext.size()) << "] = {";
+  if (pr.ext.empty()) s << "0";
+  for (size_t a = 0; a < pr.ext.size(); a++) s << (a ? "," : "") << pr.ext[a];
+  s << "};\n};\nFTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
+  return s.str();
+}
+
+static const std::vector<std::string>& rtc_options() {
+  static const std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  return o;
+}
+
+/* Compile (or fetch from the cache) the code object for a source. */
+static std::string code_object(const std::string& src, std::string* path_out) {
+  std::string key = src;
+  for (auto& o : rtc_options()) key += "\n" + o;
+  int rtcver = 0, rtcmin = 0;
+  hiprtcVersion(&rtcver, &rtcmin);
+  key += "\nhiprtc " + std::to_string(rtcver) + "." + std::to_string(rtcmin);
+  char name[64];
+  snprintf(name, sizeof name, "ftr_%016llx.co", (unsigned long long)fnv1a(key));
+  const std::string dir = cache_dir();
+  const std::string path = dir + "/" + name;
+  if (path_out) *path_out = path;
+  {
+    std::ifstream f(path, std::ios::binary);
+    if (f) {
+      std::stringstream ss;
+      ss << f.rdbuf();
+      std::string co = ss.str();
+      if (!co.empty()) return co;
+    }
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "ftr_kernel.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+    throw Error(FTR_ERR_COMPILE, "hiprtcCreateProgram failed");
+  std::vector<const char*> opts;
+  for (auto& o : rtc_options()) opts.push_back(o.c_str());
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+  if (rc != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls + 1, 0);
+    hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    throw Error(FTR_ERR_COMPILE, "kernel specialisation failed: " + log.substr(0, 4000));
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::string co(cs, 0);
+  hiprtcGetCode(prog, &co[0]);
+  hiprtcDestroyProgram(&prog);
+  /* store atomically when the cache directory is usable */
+  mkdir(dir.c_str(), 0755);
+  const std::string tmp = path + ".tmp" + std::to_string(getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    if (f) f.write(co.data(), (std::streamsize)co.size());
+  }
+  if (rename(tmp.c_str(), path.c_str()) != 0) unlink(tmp.c_str());
+  return co;
+}
+
+static std::string full_source(const ftrh::Prep& pr, const Variant& v) {
+  return std::string(ftr_embedded_device_source) + spec_source(pr, v);
+}
+
+static void fill_params(ftr_problem* p) {
+  const ftrh::Prep& pr = p->prep;
+  ftr::DevParams& d = p->hprm;
+  memset(&d, 0, sizeof d);
+  d.D = pr.D;
+  d.mhalfD = -pr.D / 2.;
+  d.W = pr.W;
+  d.mW = -pr.W;
+  d.Lf = (double)pr.L;
+  d.IGtr = pr.IGtr;
+  d.def_ref = pr.def_ref;
+  d.inv_dlw = 1. / (pr.D / 2. * pr.L + pr.W);
+  for (int e = 0; e < pr.g.E; e++) {
+    d.nu[e] = pr.g.nu[e];
+    d.m2[e] = pr.m2[e];
+  }
+  for (int i = 0; i < pr.g.V; i++)
+    for (int j = 0; j < pr.g.V; j++) d.P[i * FTR_MAX_V + j] = pr.P[(size_t)i * pr.g.V + j];
+}
+
+static void prep_from_input(ftrh::Prep& pr, const ftr_input* in) {
+  if (!in || in->E < 1) throw Error(FTR_ERR_INVALID, "input graph must have at least one edge");
+  ftrh::build_graph(pr.g, in->E, in->edges, in->nu);
+  if (in->V != pr.g.V) throw Error(FTR_ERR_INVALID, "input scalarproducts must be a (V x V)-matrix");
+  if (in->K < 1 || in->K > FTR_MAX_K) throw Error(FTR_ERR_LIMIT, "num_eps_terms must be in [1, 16]");
+  if (!(in->D > 0)) throw Error(FTR_ERR_DOMAIN, "dimension must be positive");
+  pr.P.assign(in->P, in->P + (size_t)in->V * in->V);
+  pr.m2.assign(in->m2, in->m2 + in->E);
+  pr.D = in->D;
+  pr.K = in->K;
+  pr.lambda = in->lambda;
+  if (pr.g.V > FTR_MAX_V) throw Error(FTR_ERR_LIMIT, "more than 24 vertices are not supported on the GPU path");
+  ftrh::preprocess(pr);
+  if (pr.g.E > FTR_MAX_E) throw Error(FTR_ERR_LIMIT, "more than 30 edges are not supported");
+}
+
+static void ensure_scratch(ftr_ctx* c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return;
+  if (c->d_scratch) HIPCHK(hipFree(c->d_scratch));
+  c->d_scratch = nullptr;
+  const size_t nb = std::max(bytes, (size_t)1 << 20);
+  HIPCHK(hipMalloc(&c->d_scratch, nb));
+  c->scratch_bytes = nb;
+}
+
+static Module load_module(ftr_ctx* c, const std::string& src) {
+  auto it = c->modules.find(src);
+  if (it != c->modules.end()) return it->second;
+  const std::string co = code_object(src, nullptr);
+  Module m;
+  HIPCHK(hipModuleLoadData(&m.mod, co.data()));
+  HIPCHK(hipModuleGetFunction(&m.f_int, m.mod, "ftrk_int"));
+  HIPCHK(hipModuleGetFunction(&m.f_rep, m.mod, "ftrk_rep"));
+  HIPCHK(hipModuleGetFunction(&m.f_smp, m.mod, "ftrk_smp"));
+  c->modules[src] = m;
+  return m;
+}
+
+/* Geometry + stream positioning for a run of n samples starting at sample
+ * `first` of the stream seeded with `seed`.  Writes block states and lane
+ * polynomials to the context scratch; returns (nblocks, S). */
+struct Geometry {
+  uint64_t nblocks = 0, S = 0, lanes = 0;
+  uint64_t* d_states = nullptr;
+  uint64_t* d_polys = nullptr;
+  double* d_partials = nullptr;
+};
+
+static Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req,
+                             int block, int nq) {
+  ftr_ctx* c = p->ctx;
+  const int E = p->prep.g.E;
+  Geometry G;
+  uint64_t lanes_target = lanes_req;
+  if (!lanes_target) {
+    int nbpc = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, p->m.f_int, block, p->lds) != hipSuccess || nbpc < 1)
+      nbpc = 1;
+    lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * block;
+  }
+  const uint64_t Smin = 4;
+  uint64_t S = (n + lanes_target - 1) / lanes_target;
+  if (S < Smin) S = Smin;
+  const uint64_t lanes = (n + S - 1) / S;
+  const uint64_t nblocks = (lanes + block - 1) / block;
+  G.nblocks = nblocks;
+  G.S = S;
+  G.lanes = lanes;
+  const uint64_t per = 2 * (uint64_t)E - 1;
+  int lb = 0;
+  while ((1 << lb) < block) lb++;
+  std::vector<uint64_t> states(4 * nblocks), polys(4 * (size_t)lb);
+  uint64_t st[4];
+  ftrh::xo_seed(seed, st);
+  {
+    const unsigned __int128 d0 = (unsigned __int128)first * per;
+    if (d0) ftrh::xo_apply(ftrh::xo_jump_poly((uint64_t)d0, (uint64_t)(d0 >> 64)), st);
+  }
+  const unsigned __int128 dl = (unsigned __int128)S * per;
+  ftrh::Poly q = ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64));
+  for (int j = 0; j < lb; j++) {
+    memcpy(&polys[4 * j], q.w, 32);
+    q = ftrh::xo_poly_square(q);
+  }
+  /* q is now the jump polynomial for one block (S * per * block) */
+  for (uint64_t b = 0; b < nblocks; b++) {
+    memcpy(&states[4 * b], st, 32);
+    if (b + 1 < nblocks) ftrh::xo_apply(q, st);
+  }
+  const size_t bytes_states = states.size() * 8, bytes_polys = std::max<size_t>(polys.size() * 8, 32);
+  const size_t bytes_part = (size_t)nblocks * nq * 8;
+  ensure_scratch(c, bytes_states + bytes_polys + bytes_part + 256);
+  char* base = (char*)c->d_scratch;
+  G.d_states = (uint64_t*)base;
+  G.d_polys = (uint64_t*)(base + ((bytes_states + 255) & ~(size_t)255));
+  G.d_partials = (double*)((char*)G.d_polys + ((bytes_polys + 255) & ~(size_t)255));
+  HIPCHK(hipMemcpyHtoDAsync(G.d_states, states.data(), bytes_states, c->stream));
+  if (!polys.empty()) HIPCHK(hipMemcpyHtoDAsync(G.d_polys, polys.data(), polys.size() * 8, c->stream));
+  return G;
+}
+
+static int fail(ftr_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+/* --------------------------------------------------------------- C ABI -- */
+extern "C" {
+
+int ftr_create(int device, ftr_ctx** out) {
+  if (!out) return FTR_ERR_INVALID;
+  *out = nullptr;
+  ftr_ctx* c = new ftr_ctx();
+  try {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error(FTR_ERR_DEVICE, "no such HIP device");
+    c->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipGetDeviceProperties(&c->prop, device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+  } catch (Error& e) {
+    fprintf(stderr, "feyntrop_amd: %s\n", e.what());
+    delete c;
+    return e.code;
+  }
+  *out = c;
+  return FTR_OK;
+}
+
+void ftr_destroy(ftr_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  for (auto& kv : c->modules) (void)hipModuleUnload(kv.second.mod);
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* ftr_last_error(const ftr_ctx* c) { return c ? c->err.c_str() : g_host_err.c_str(); }
+
+int ftr_set_cache_dir(const char* dir) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_cache_dir = dir ? dir : "";
+  return FTR_OK;
+}
+
+int ftr_precompile(const ftr_input* in, char* err, int errlen) {
+  try {
+    ftrh::Prep pr;
+    prep_from_input(pr, in);
+    /* both LDS limits a gfx950 part may report */
+    for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
+      const Variant v = choose_variant(pr, lim);
+      code_object(full_source(pr, v), nullptr);
+    }
+  } catch (std::exception& e) {
+    if (err && errlen > 0) {
+      strncpy(err, e.what(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fe ? fe->code : FTR_ERR_INVALID;
+  }
+  return FTR_OK;
+}
+
+int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
+  if (!out) return FTR_ERR_INVALID;
+  *out = nullptr;
+  ftr_problem* p = new ftr_problem();
+  p->ctx = c;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    prep_from_input(p->prep, in);
+    if (!c) {
+      /* host-only preparation: validation, regime, J-table */
+      p->prep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      *out = p;
+      return FTR_OK;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock);
+    p->lds = variant_lds(p->var, p->prep.g.E);
+    p->m = load_module(c, full_source(p->prep, p->var));
+    if (p->lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
+    fill_params(p);
+    const size_t NT = (size_t)1 << p->prep.g.E;
+    std::vector<double> w(NT), ir(NT);
+    std::vector<uint8_t> mml(NT);
+    for (size_t i = 0; i < NT; i++) {
+      w[i] = p->prep.Jr[i] / p->prep.r[i];
+      ir[i] = 1. / p->prep.r[i];
+      mml[i] = (uint8_t)((p->prep.mm[i] ? 1 : 0) | (p->prep.Lt[i] << 1));
+    }
+    HIPCHK(hipMalloc((void**)&p->d_prm, sizeof(ftr::DevParams)));
+    HIPCHK(hipMalloc((void**)&p->d_w, NT * 8));
+    HIPCHK(hipMalloc((void**)&p->d_jr, NT * 8));
+    HIPCHK(hipMalloc((void**)&p->d_ir, NT * 8));
+    HIPCHK(hipMalloc((void**)&p->d_mml, NT));
+    HIPCHK(hipMemcpyHtoDAsync(p->d_prm, &p->hprm, sizeof(ftr::DevParams), c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(p->d_w, w.data(), NT * 8, c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(p->d_jr, p->prep.Jr.data(), NT * 8, c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(p->d_ir, ir.data(), NT * 8, c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(p->d_mml, mml.data(), NT, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    p->prep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    int code = fe ? fe->code : FTR_ERR_INVALID;
+    ftr_problem_free(p);
+    if (!c) g_host_err = e.what();
+    return fail(c, code, e.what());
+  }
+  *out = p;
+  return FTR_OK;
+}
+
+void ftr_problem_free(ftr_problem* p) {
+  if (!p) return;
+  if (p->ctx) (void)hipSetDevice(p->ctx->device);
+  if (p->d_prm) (void)hipFree(p->d_prm);
+  if (p->d_w) (void)hipFree(p->d_w);
+  if (p->d_jr) (void)hipFree(p->d_jr);
+  if (p->d_ir) (void)hipFree(p->d_ir);
+  if (p->d_mml) (void)hipFree(p->d_mml);
+  delete p;
+}
+
+int ftr_problem_info(const ftr_problem* p, ftr_info* o) {
+  if (!p || !o) return FTR_ERR_INVALID;
+  const ftrh::Prep& pr = p->prep;
+  o->E = pr.g.E;
+  o->V = pr.g.V;
+  o->L = pr.L;
+  o->K = pr.K;
+  o->W = pr.W;
+  o->IGtr = pr.IGtr;
+  o->euclidean = pr.euclidean;
+  o->pseudo_euclidean = pr.pseudo;
+  o->generic = pr.generic;
+  o->gp_property = pr.gp;
+  o->lambda_eff = pr.lambda_eff;
+  o->path = pr.path;
+  o->seconds_preprocessing = pr.seconds;
+  return FTR_OK;
+}
+
+int ftr_jtable(const ftr_problem* p, double* Jr, double* r, uint8_t* mm, uint8_t* L) {
+  if (!p) return FTR_ERR_INVALID;
+  const size_t NT = p->prep.Jr.size();
+  if (Jr) memcpy(Jr, p->prep.Jr.data(), NT * 8);
+  if (r) memcpy(r, p->prep.r.data(), NT * 8);
+  if (mm) memcpy(mm, p->prep.mm.data(), NT);
+  if (L) memcpy(L, p->prep.Lt.data(), NT);
+  return FTR_OK;
+}
+
+int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts, ftr_moments* out) {
+  if (!c) {
+    g_host_err = "no HIP device context: the problem was prepared host-only";
+    return FTR_ERR_DEVICE;
+  }
+  if (!p || !out) return FTR_ERR_INVALID;
+  if (!p->m.f_int) return fail(c, FTR_ERR_INVALID, "problem was prepared without a device context");
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(c->device));
+    const int K = p->prep.K, NQ = 2 + 4 * K;
+    memset(out, 0, sizeof *out);
+    out->K = K;
+    if (opts && opts->block && opts->block != p->var.block)
+      throw Error(FTR_ERR_INVALID, "block size differs from the compiled variant");
+    if (N > 0) {
+      Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ);
+      ftr::Launch a;
+      a.prm = p->d_prm;
+      a.t_w = p->d_w;
+      a.t_jr = p->d_jr;
+      a.t_ir = p->d_ir;
+      a.t_mmL = p->d_mml;
+      a.block_state = G.d_states;
+      a.lane_poly = G.d_polys;
+      a.N = N;
+      a.S = G.S;
+      a.partials = G.d_partials;
+      void* args[] = {&a};
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      HIPCHK(hipModuleLaunchKernel(p->m.f_int, (unsigned)G.nblocks, 1, 1, p->var.block, 1, 1, (unsigned)p->lds,
+                                   c->stream, args, nullptr));
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      c->h_partials.resize((size_t)G.nblocks * NQ);
+      HIPCHK(hipMemcpyDtoHAsync(c->h_partials.data(), G.d_partials, c->h_partials.size() * 8, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      out->kernel_ms = ms;
+      out->lanes = G.lanes;
+      out->samples_per_lane = G.S;
+      std::vector<double> tot(NQ, 0.0);
+      for (uint64_t b = 0; b < G.nblocks; b++)
+        for (int q = 0; q < NQ; q++) tot[q] += c->h_partials[b * NQ + q];
+      out->dropped = (uint64_t)tot[1];
+      for (int k = 0; k < K; k++) {
+        out->n[k][0] = tot[0];
+        out->s1[k][0] = tot[2 + 4 * k];
+        out->s2[k][0] = tot[3 + 4 * k];
+        out->n[k][1] = tot[0];
+        out->s1[k][1] = tot[4 + 4 * k];
+        out->s2[k][1] = tot[5 + 4 * k];
+      }
+    }
+    if (p->prep.path != 2)
+      for (int k = 0; k < K; k++) {
+        /* the reference's two zero updates of the imaginary stats */
+        out->n[k][1] = 2;
+        out->s1[k][1] = 0;
+        out->s2[k][1] = 0;
+      }
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return FTR_OK;
+}
+
+int ftr_finalize(const ftr_problem* p, const ftr_moments* m, double* res) {
+  if (!p || !m || !res) return FTR_ERR_INVALID;
+  for (int k = 0; k < m->K; k++)
+    for (int q = 0; q < 2; q++) {
+      const double N = m->n[k][q];
+      const double avg = m->s1[k][q] / N;
+      const double var = (N / (N - 1)) * (m->s2[k][q] / N - std::pow(avg, 2));
+      res[4 * k + 2 * q] = avg;
+      res[4 * k + 2 * q + 1] = std::sqrt(var / N);
+    }
+  return FTR_OK;
+}
+
+int ftr_sample(ftr_ctx* c, ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, double* X_dev,
+               double* psixi_dev) {
+  if (!c || !p) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    if (n == 0) return FTR_OK;
+    Geometry G = setup_stream(p, first, n, seed, 0, p->var.block, 2 + 4 * p->prep.K);
+    ftr::Launch a;
+    memset(&a, 0, sizeof a);
+    a.prm = p->d_prm;
+    a.t_w = p->d_w;
+    a.t_jr = p->d_jr;
+    a.t_ir = p->d_ir;
+    a.t_mmL = p->d_mml;
+    a.block_state = G.d_states;
+    a.lane_poly = G.d_polys;
+    a.N = n;
+    a.S = G.S;
+    a.partials = G.d_partials;
+    void* args[] = {&a, &X_dev, &psixi_dev};
+    HIPCHK(hipModuleLaunchKernel(p->m.f_smp, (unsigned)G.nblocks, 1, 1, p->var.block, 1, 1, (unsigned)p->lds,
+                                 c->stream, args, nullptr));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return FTR_OK;
+}
+
+int ftr_replay(ftr_ctx* c, ftr_problem* p, uint64_t n, const double* X_dev, const double* psixi_dev,
+               double* out_dev, uint8_t* dropped_dev) {
+  if (!c || !p) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    if (n == 0) return FTR_OK;
+    const ftr::DevParams* prm = p->d_prm;
+    void* args[] = {(void*)&prm, &n, (void*)&X_dev, (void*)&psixi_dev, &out_dev, &dropped_dev};
+    const uint64_t nb = (n + 255) / 256;
+    HIPCHK(hipModuleLaunchKernel(p->m.f_rep, (unsigned)nb, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return FTR_OK;
+}
+
+void ftr_free_string(char* s) { free(s); }
+
+}  // extern "C"
+
+const std::vector<std::string>& ftr_problem_log(const ftr_problem* p) { return p->prep.log; }

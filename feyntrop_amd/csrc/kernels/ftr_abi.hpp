@@ -1,0 +1,51 @@
+/*
+ * ftr_abi.hpp -- data layout shared by the host runtime and the HIP kernels
+ * (plain C++: no HIP types).  Limits: E <= FTR_MAX_E edges (the 2^E J-table
+ * is indexed by 32-bit masks), V <= FTR_MAX_V vertices.
+ */
+#pragma once
+
+#ifdef __HIPCC_RTC__
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+#else
+#include <stdint.h>
+#endif
+
+#define FTR_MAX_E 30
+#define FTR_MAX_V 24
+
+namespace ftr {
+
+/* ----------------------------------------------------------- parameters -- */
+/* Uniform per-problem data, read by every lane (scalar loads). */
+struct DevParams {
+  double D;          /* dimension */
+  double mhalfD;     /* -D/2 */
+  double W;          /* superficial degree of divergence */
+  double mW;         /* -W */
+  double Lf;         /* loop number as double */
+  double IGtr;       /* J(G) */
+  double def_ref;    /* -lambda_eff (0 on the real path) */
+  double inv_dlw;    /* 1 / (D/2 L + W) */
+  double nu[FTR_MAX_E];
+  double m2[FTR_MAX_E];
+  double P[FTR_MAX_V * FTR_MAX_V];
+};
+
+/* Kernel arguments for one problem. */
+struct Launch {
+  const DevParams* prm;
+  const double* t_w;       /* J(g)/r(g), 2^E */
+  const double* t_jr;      /* J(g), 2^E */
+  const double* t_ir;      /* 1/r(g), 2^E */
+  const uint8_t* t_mmL;    /* mm | L<<1, 2^E */
+  const uint64_t* block_state; /* nblocks x 4: stream state at the block's first sample */
+  const uint64_t* lane_poly;   /* log2(blockDim) x 4: jump polynomials for S*(2E-1)*2^j */
+  uint64_t N;              /* total samples */
+  uint64_t S;              /* samples per lane */
+  double* partials;        /* nblocks x (2 + 4K) */
+};
+
+}  // namespace ftr

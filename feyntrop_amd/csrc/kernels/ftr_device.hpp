@@ -1,0 +1,659 @@
+/*
+ * ftr_device.hpp -- HIP device code for the tropical sampling hot path
+ * (gfx950, FP64).  One sample per lane; every graph-dependent index is a
+ * compile-time constant of the graph spec G, so all per-sample matrices live
+ * in registers.
+ *
+ * The file is included by the ahead-of-time kernel units and is also embedded
+ * verbatim in the runtime library for hipRTC specialisation of new graphs.
+ *
+ * Graph spec G (generated per graph):
+ *   static constexpr int E, V, NEXT, LOOPS;     edges, vertices, |ext|, loops
+ *   static constexpr int hi[E], lo[E];           edge endpoints, hi > lo
+ *   static constexpr int ext[NEXT];              vertices with nonzero P rows
+ *
+ * Reference semantics followed (michibo/feyntrop):
+ *   sampler   srcs/tropical_sampling.hpp:468-564
+ *   integrand srcs/feynman_integral.hpp:166-362, srcs/laplacian.hpp:14-215
+ *   RNG       srcs/random.hpp:16-140
+ * Differences in *method* (same mathematics, different rounding):
+ *   - the deleted vertex v* (feynman_integral.hpp:166-194) is kept in place and
+ *     its row/column are replaced by the identity, so the V x V Laplacian has
+ *     static structure; det and inverse of the reduced matrix are unchanged;
+ *   - the real Laplacian (SPD) is factorised by LDL^T in static order without
+ *     pivoting instead of Eigen's diagonally pivoted LDLT;
+ *   - the complex Laplacian La(1/z) and the deformation Jacobian are factorised
+ *     as complex-symmetric LDL^T without pivoting: det(I + i c H X) =
+ *     det(X^-1 + i c H) * prod x, whose real part diag(1/x) is positive
+ *     definite (Eigen: FullPivLU / PartialPivLU);
+ *   - tr(La^-1 P) is taken over the external vertices only (P is zero on the
+ *     others) from column solves.
+ */
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#endif
+#include "ftr_abi.hpp"
+
+namespace ftr {
+
+/* ----------------------------------------------------------------- RNG -- */
+struct Xoshiro {
+  uint64_t s0, s1, s2, s3;
+  __host__ __device__ inline uint64_t next() {
+    const uint64_t res = s0 + s3;
+    const uint64_t t = s1 << 17;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = (s3 << 45) | (s3 >> 19);
+    return res;
+  }
+  /* random.hpp:16-27 */
+  __host__ __device__ inline double uniform() {
+    return (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+  }
+};
+
+/* state <- q(M) state for a jump polynomial q (4 words, bit i = x^i). */
+__device__ inline void apply_poly(Xoshiro& g, const uint64_t* q) {
+  Xoshiro acc{0, 0, 0, 0};
+  for (int w = 0; w < 4; w++) {
+    const uint64_t bits = q[w];
+    for (int b = 0; b < 64; b++) {
+      if ((bits >> b) & 1) { acc.s0 ^= g.s0; acc.s1 ^= g.s1; acc.s2 ^= g.s2; acc.s3 ^= g.s3; }
+      g.next();
+    }
+  }
+  g = acc;
+}
+
+/* ------------------------------------------------------------- complex -- */
+struct cplx {
+  double re, im;
+};
+__host__ __device__ inline cplx cmk(double a, double b) { return cplx{a, b}; }
+__host__ __device__ inline cplx operator+(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cplx operator-(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cplx operator-(cplx a) { return cplx{-a.re, -a.im}; }
+__host__ __device__ inline cplx operator*(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cplx operator*(double s, cplx a) { return cplx{s * a.re, s * a.im}; }
+__host__ __device__ inline cplx crecip(cplx a) {
+  const double inv = 1.0 / (a.re * a.re + a.im * a.im);
+  return cplx{a.re * inv, -a.im * inv};
+}
+/* principal log: (log|z|, arg z) */
+__host__ __device__ inline cplx clog_(cplx z) {
+  return cplx{log(hypot(z.re, z.im)), atan2(z.im, z.re)};
+}
+/* libstdc++ pow(complex<double>, double): real fast path for positive reals,
+ * else polar(exp(y Re log z), y Im log z)  (complex:1028-1039) */
+__host__ __device__ inline cplx cpow_real(cplx z, double y) {
+  if (z.im == 0.0 && z.re > 0.0) return cplx{pow(z.re, y), 0.0};
+  const cplx t = clog_(z);
+  const double rho = exp(y * t.re);
+  double s, c;
+  sincos(y * t.im, &s, &c);
+  return cplx{rho * c, rho * s};
+}
+__host__ __device__ inline bool cfinite(cplx z) { return __builtin_isfinite(z.re) && __builtin_isfinite(z.im); }
+
+/* lower-triangular packed index for i >= j */
+#define FTR_TRI(i, j) ((i) * ((i) + 1) / 2 + (j))
+
+/* ------------------------------------------------------------- sampler -- */
+/* tropical_sampling.hpp:503-564.  Table access goes through the TableView
+ * (LDS or global).  Returns psi_tr, xi_tr; X[e] filled in edge order. */
+template <class G, class TV>
+__device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& psi_tr,
+                                    double& xi_tr) {
+  constexpr int E = G::E;
+  uint32_t g = (E == 32) ? 0xffffffffu : ((1u << E) - 1u);
+  double JJr = tv.jr(g);
+  uint32_t b0 = tv.mmL(g);
+  bool MM = b0 & 1;
+  int oL = (int)(b0 >> 1);
+  double Gv = 0.0, F = 1.0, XX = 1.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) X[e] = 0.0;
+  for (int j = 0; j < E; j++) {
+    const double R = rng.uniform() * JJr;
+    double T = 0.0;
+    int e = 0;
+    uint32_t m = g;
+    while (m) {
+      const int q = __builtin_ctz(m);
+      m &= m - 1;
+      T += tv.w(g ^ (1u << q));
+      e = q;
+      if (T > R) break;
+    }
+    const uint32_t c = g ^ (1u << e);
+    JJr = tv.jr(c);
+    const double ir = tv.ir(c);
+    const uint32_t b = tv.mmL(c);
+    const bool mm = b & 1;
+    const int L = (int)(b >> 1);
+    g = c;
+    if (MM && !mm) Gv = XX;
+    MM = mm;
+#pragma unroll
+    for (int k = 0; k < E; k++)
+      if (k == e) X[k] = XX;
+    if (L < oL) {
+      F *= XX;
+      oL = L;
+    }
+    if (j < E - 1) {
+      const double U = rng.uniform();
+      XX *= pow(1. - U, ir);
+    }
+  }
+  psi_tr = F;
+  xi_tr = Gv * F;
+}
+
+/* ----------------------------------------------------------- integrand -- */
+/* out[k] (complex, imag 0 on the real path).  Returns false when dropped
+ * (non-finite R0, feynman_integral.hpp:227-234, :254-261, :329-347). */
+template <class G, int K, int MODE>
+__host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E], double psi_tr,
+                                          double xi_tr, cplx (&out)[K]) {
+  constexpr int E = G::E, V = G::V, NT = V * (V + 1) / 2;
+  /* vertex weights and v* (first minimum) -- feynman_integral.hpp:166-194 */
+  double XV[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) XV[v] = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    XV[G::hi[e]] += X[e];
+    XV[G::lo[e]] += X[e];
+  }
+  int vs = 0;
+  double mv = XV[0];
+#pragma unroll
+  for (int v = 1; v < V; v++)
+    if (mv > XV[v]) {
+      mv = XV[v];
+      vs = v;
+    }
+  /* projective rescale -- :201-207 */
+  const double tgt = pow(psi_tr, p.mhalfD) * pow(psi_tr / xi_tr, p.W);
+  const double scale = pow(tgt, p.inv_dlw);
+  double Xi[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    X[e] *= scale;
+    Xi[e] = 1.0 / X[e];
+  }
+  /* Laplacian with weights 1/x (laplacian.hpp:14-67), v* row/col -> identity */
+  double M[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) M[q] = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    M[FTR_TRI(G::lo[e], G::lo[e])] += Xi[e];
+    M[FTR_TRI(G::hi[e], G::hi[e])] += Xi[e];
+    M[FTR_TRI(G::hi[e], G::lo[e])] -= Xi[e];
+  }
+#pragma unroll
+  for (int i = 0; i < V; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++)
+      if (i == vs || j == vs) M[FTR_TRI(i, j)] = (i == j) ? 1.0 : 0.0;
+  /* LDL^T, static order; M keeps L below the diagonal, dinv the inverse pivots */
+  double dprod = 1.0, dinv[V];
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    const double d = M[FTR_TRI(k, k)];
+    dprod *= d;
+    const double inv = 1.0 / d;
+    dinv[k] = inv;
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      const double aik = M[FTR_TRI(i, k)];
+#pragma unroll
+      for (int j = k + 1; j <= i; j++) M[FTR_TRI(i, j)] -= aik * (M[FTR_TRI(j, k)] * inv);
+    }
+#pragma unroll
+    for (int i = k + 1; i < V; i++) M[FTR_TRI(i, k)] *= inv;
+  }
+  double xprod = 1.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) xprod *= X[e];
+  const double psi = dprod * xprod;
+
+  if constexpr (MODE != 2) {
+    double R = p.IGtr * pow(psi, p.mhalfD);
+    if (!__builtin_isfinite(R)) return false;
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = cmk(0.0, 0.0);
+    if constexpr (MODE == 1) {
+      /* W = 0 and one order: no second Symanzik needed (:237-241) */
+      out[0] = cmk(R, 0.0);
+      return true;
+    }
+    /* tr(La^-1 P) over the external vertices from column solves */
+    double tr = 0.0;
+#pragma unroll
+    for (int a = 0; a < G::NEXT; a++) {
+      const int j = G::ext[a];
+      double y[V];
+#pragma unroll
+      for (int i = 0; i < V; i++) y[i] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int i = 0; i < V; i++)
+#pragma unroll
+        for (int q = 0; q < i; q++) y[i] -= M[FTR_TRI(i, q)] * y[q];
+#pragma unroll
+      for (int i = 0; i < V; i++) y[i] *= dinv[i];
+#pragma unroll
+      for (int i = V - 1; i >= 0; i--)
+#pragma unroll
+        for (int q = i + 1; q < V; q++) y[i] -= M[FTR_TRI(q, i)] * y[q];
+      double colsum = 0.0;
+#pragma unroll
+      for (int b = 0; b < G::NEXT; b++) colsum += y[G::ext[b]] * p.P[G::ext[b] * FTR_MAX_V + j];
+      tr += (j == vs) ? 0.0 : colsum;
+    }
+    double dot = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) dot += p.m2[e] * X[e];
+    const double pxi = -tr + dot;
+    R *= pow(pxi, p.mW);
+    if (!__builtin_isfinite(R)) return false;
+    out[0] = cmk(R, 0.0);
+    if (K > 1) {
+      const double ef = log(psi) - p.Lf * log(pxi);
+#pragma unroll
+      for (int k = 1; k < K; k++) {
+        R *= ef / (double)k;
+        out[k] = cmk(R, 0.0);
+      }
+    }
+    return true;
+  } else {
+    /* ---- analytic continuation (feynman_integral.hpp:278-362) ---- */
+    /* Gm = reduced inverse, zero on v*: Gm = L^-T D^-1 L^-1 */
+    double Li[NT]; /* L^-1, unit lower (diagonal unused) */
+#pragma unroll
+    for (int i = 0; i < V; i++)
+#pragma unroll
+      for (int j = 0; j < i; j++) {
+        double s = -M[FTR_TRI(i, j)];
+#pragma unroll
+        for (int q = j + 1; q < i; q++) s -= M[FTR_TRI(i, q)] * Li[FTR_TRI(q, j)];
+        Li[FTR_TRI(i, j)] = s;
+      }
+    double Gm[NT];
+#pragma unroll
+    for (int i = 0; i < V; i++)
+#pragma unroll
+      for (int j = 0; j <= i; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = i; k < V; k++) {
+          const double lki = (k == i) ? 1.0 : Li[FTR_TRI(k, i)];
+          const double lkj = (k == j) ? 1.0 : Li[FTR_TRI(k, j)];
+          s += lki * lkj * dinv[k];
+        }
+        Gm[FTR_TRI(i, j)] = (i == vs || j == vs) ? 0.0 : s;
+      }
+    /* LPGPLT = Gm P Gm over the external support */
+    double Y[V][G::NEXT];
+#pragma unroll
+    for (int u = 0; u < V; u++)
+#pragma unroll
+      for (int b = 0; b < G::NEXT; b++) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < G::NEXT; a++) {
+          const int i = G::ext[a];
+          const double gui = (u >= i) ? Gm[FTR_TRI(u, i)] : Gm[FTR_TRI(i, u)];
+          s += gui * p.P[i * FTR_MAX_V + G::ext[b]];
+        }
+        Y[u][b] = s;
+      }
+    double LP[NT];
+#pragma unroll
+    for (int u = 0; u < V; u++)
+#pragma unroll
+      for (int w = 0; w <= u; w++) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < G::NEXT; b++) {
+          const int j = G::ext[b];
+          const double gjw = (j >= w) ? Gm[FTR_TRI(j, w)] : Gm[FTR_TRI(w, j)];
+          s += Y[u][b] * gjw;
+        }
+        LP[FTR_TRI(u, w)] = s;
+      }
+#define FTR_SYM(A, i, j) (((i) >= (j)) ? A[FTR_TRI(i, j)] : A[FTR_TRI(j, i)])
+    /* first and second derivatives of pphi (laplacian.hpp:102-215) */
+    double th[E];
+    double dd[E * (E + 1) / 2];
+#pragma unroll
+    for (int a = 0; a < E; a++) {
+      const int ka = G::hi[a], la = G::lo[a];
+      const double s = FTR_SYM(LP, ka, ka) + FTR_SYM(LP, la, la) - 2.0 * FTR_SYM(LP, ka, la);
+      const double t = FTR_SYM(Gm, ka, ka) + FTR_SYM(Gm, la, la) - 2.0 * FTR_SYM(Gm, ka, la);
+      const double xa = Xi[a] * Xi[a];
+      const double xas = xa * s;
+      th[a] = -xas;
+      dd[FTR_TRI(a, a)] = 2.0 * xas * (Xi[a] - xa * t);
+#pragma unroll
+      for (int b = 0; b < a; b++) {
+        const int kb = G::hi[b], lb = G::lo[b];
+        const double s2 = FTR_SYM(LP, ka, kb) + FTR_SYM(LP, la, lb) - FTR_SYM(LP, ka, lb) - FTR_SYM(LP, la, kb);
+        const double t2 = FTR_SYM(Gm, ka, kb) + FTR_SYM(Gm, la, lb) - FTR_SYM(Gm, ka, lb) - FTR_SYM(Gm, la, kb);
+        const double xb = Xi[b] * Xi[b];
+        dd[FTR_TRI(a, b)] = -2.0 * xb * xa * s2 * t2;
+      }
+    }
+    /* deformation: theta = def_ref (d pphi + m^2), z = x e^{i theta} */
+    cplx cX[E], cXi[E];
+    double xne = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      th[e] = (th[e] + p.m2[e]) * p.def_ref;
+      double sn, cs;
+      sincos(th[e], &sn, &cs);
+      cX[e] = cmk(X[e] * cs, X[e] * sn);
+      cXi[e] = cmk(Xi[e] * cs, -Xi[e] * sn);
+      xne += p.nu[e] * th[e];
+    }
+    /* complex Laplacian La(1/z), v* -> identity, complex-symmetric LDL^T */
+    cplx CM[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) CM[q] = cmk(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      CM[FTR_TRI(G::lo[e], G::lo[e])] = CM[FTR_TRI(G::lo[e], G::lo[e])] + cXi[e];
+      CM[FTR_TRI(G::hi[e], G::hi[e])] = CM[FTR_TRI(G::hi[e], G::hi[e])] + cXi[e];
+      CM[FTR_TRI(G::hi[e], G::lo[e])] = CM[FTR_TRI(G::hi[e], G::lo[e])] - cXi[e];
+    }
+#pragma unroll
+    for (int i = 0; i < V; i++)
+#pragma unroll
+      for (int j = 0; j <= i; j++)
+        if (i == vs || j == vs) CM[FTR_TRI(i, j)] = cmk((i == j) ? 1.0 : 0.0, 0.0);
+    cplx cdet = cmk(1.0, 0.0), cdinv[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+      const cplx d = CM[FTR_TRI(k, k)];
+      cdet = cdet * d;
+      const cplx inv = crecip(d);
+      cdinv[k] = inv;
+#pragma unroll
+      for (int i = k + 1; i < V; i++) {
+        const cplx lik = CM[FTR_TRI(i, k)] * inv;
+#pragma unroll
+        for (int j = k + 1; j <= i; j++) CM[FTR_TRI(i, j)] = CM[FTR_TRI(i, j)] - lik * CM[FTR_TRI(j, k)];
+      }
+#pragma unroll
+      for (int i = k + 1; i < V; i++) CM[FTR_TRI(i, k)] = CM[FTR_TRI(i, k)] * inv;
+    }
+    cplx cxp = cmk(1.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < E; e++) cxp = cxp * cX[e];
+    const cplx cPsi = cxp * cdet;
+    cplx cR = p.IGtr * cpow_real(cPsi, p.mhalfD);
+    /* deformation Jacobian: det(I + i c H X) = det(X^-1 + i c H) prod x */
+    cplx B[E * (E + 1) / 2];
+#pragma unroll
+    for (int a = 0; a < E; a++)
+#pragma unroll
+      for (int b = 0; b <= a; b++)
+        B[FTR_TRI(a, b)] = cmk((a == b) ? Xi[a] : 0.0, p.def_ref * dd[FTR_TRI(a, b)]);
+    cplx jdet = cmk(1.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+      const cplx d = B[FTR_TRI(k, k)];
+      jdet = jdet * d;
+      const cplx inv = crecip(d);
+#pragma unroll
+      for (int i = k + 1; i < E; i++) {
+        const cplx lik = B[FTR_TRI(i, k)] * inv;
+#pragma unroll
+        for (int j = k + 1; j <= i; j++) B[FTR_TRI(i, j)] = B[FTR_TRI(i, j)] - lik * B[FTR_TRI(j, k)];
+      }
+    }
+    jdet = xprod * jdet;
+    double sn, cs;
+    sincos(xne, &sn, &cs);
+    cR = cR * (jdet * cmk(cs, sn));
+    /* tr(La(1/z)^-1 P) over the external support */
+    cplx ctr = cmk(0.0, 0.0);
+#pragma unroll
+    for (int a = 0; a < G::NEXT; a++) {
+      const int j = G::ext[a];
+      cplx y[V];
+#pragma unroll
+      for (int i = 0; i < V; i++) y[i] = cmk((i == j) ? 1.0 : 0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < V; i++)
+#pragma unroll
+        for (int q = 0; q < i; q++) y[i] = y[i] - CM[FTR_TRI(i, q)] * y[q];
+#pragma unroll
+      for (int i = 0; i < V; i++) y[i] = y[i] * cdinv[i];
+#pragma unroll
+      for (int i = V - 1; i >= 0; i--)
+#pragma unroll
+        for (int q = i + 1; q < V; q++) y[i] = y[i] - CM[FTR_TRI(q, i)] * y[q];
+      cplx colsum = cmk(0.0, 0.0);
+#pragma unroll
+      for (int b = 0; b < G::NEXT; b++) colsum = colsum + p.P[G::ext[b] * FTR_MAX_V + j] * y[G::ext[b]];
+      if (j != vs) ctr = ctr + colsum;
+    }
+    cplx cdot = cmk(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < E; e++) cdot = cdot + p.m2[e] * cX[e];
+    const cplx cPXi = cdot - ctr;
+    cR = cR * cpow_real(cPXi, p.mW);
+    if (!cfinite(cR)) return false;
+    out[0] = cR;
+    if (K > 1) {
+      const cplx lp = clog_(cPsi), lx = clog_(cPXi);
+      const cplx ef = lp - p.Lf * lx;
+#pragma unroll
+      for (int k = 1; k < K; k++) {
+        cR = cR * cmk(ef.re / (double)k, ef.im / (double)k);
+        out[k] = cR;
+      }
+    }
+    return true;
+#undef FTR_SYM
+  }
+}
+
+/* -------------------------------------------------------- table views -- */
+/* TMODE 0: whole table in LDS; 1: w = J/r in LDS, the rest from global
+ * (L2-resident); 2: everything from global. */
+template <int TMODE>
+struct TableView {
+  const double* pw;
+  const double* pjr;
+  const double* pir;
+  const uint8_t* pmm;
+  __device__ inline double w(uint32_t g) const { return pw[g]; }
+  __device__ inline double jr(uint32_t g) const { return pjr[g]; }
+  __device__ inline double ir(uint32_t g) const { return pir[g]; }
+  __device__ inline uint32_t mmL(uint32_t g) const { return pmm[g]; }
+};
+
+template <int TMODE>
+__host__ __device__ constexpr size_t table_lds_bytes(int E) {
+  return TMODE == 0 ? ((size_t)25 << E) + 64 : (TMODE == 1 ? ((size_t)8 << E) : 0);
+}
+
+/* Stage the table into LDS and build the view. */
+template <int TMODE>
+__device__ inline TableView<TMODE> stage_table(const Launch& a, int E, char* lds) {
+  TableView<TMODE> tv;
+  const uint32_t NT = 1u << E;
+  if constexpr (TMODE == 2) {
+    tv.pw = a.t_w; tv.pjr = a.t_jr; tv.pir = a.t_ir; tv.pmm = a.t_mmL;
+    return tv;
+  } else {
+    double* w = reinterpret_cast<double*>(lds);
+    for (uint32_t i = threadIdx.x; i < NT; i += blockDim.x) w[i] = a.t_w[i];
+    tv.pw = w;
+    if constexpr (TMODE == 0) {
+      double* jr = w + NT;
+      double* ir = jr + NT;
+      uint8_t* mm = reinterpret_cast<uint8_t*>(ir + NT);
+      for (uint32_t i = threadIdx.x; i < NT; i += blockDim.x) {
+        jr[i] = a.t_jr[i];
+        ir[i] = a.t_ir[i];
+        mm[i] = a.t_mmL[i];
+      }
+      tv.pjr = jr; tv.pir = ir; tv.pmm = mm;
+    } else {
+      tv.pjr = a.t_jr; tv.pir = a.t_ir; tv.pmm = a.t_mmL;
+    }
+    __syncthreads();
+    return tv;
+  }
+}
+
+/* ------------------------------------------------------ reductions -- */
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+/* Lane stream positioning: the block's first state comes from the host;
+ * lane t applies the jump polynomials of the set bits of t. */
+__device__ inline Xoshiro lane_stream(const Launch& a) {
+  const uint64_t* bs = a.block_state + 4 * (size_t)blockIdx.x;
+  Xoshiro g{bs[0], bs[1], bs[2], bs[3]};
+  const unsigned t = threadIdx.x;
+  for (int j = 0; (1u << j) < blockDim.x; j++) {
+    Xoshiro h = g;
+    apply_poly(h, a.lane_poly + 4 * j);
+    if ((t >> j) & 1) g = h;
+  }
+  return g;
+}
+
+/* Main body: each lane draws its contiguous run of samples from the single
+ * stream, evaluates the integrand and accumulates (n, drops, S1, S2) per
+ * order; the block writes one row of partial moments
+ * [n, drops, (s1re, s2re, s1im, s2im) x K]. */
+template <class G, int K, int MODE, int TMODE>
+__device__ inline void integrate_body(const Launch& a) {
+  extern __shared__ __align__(16) char lds[];
+  constexpr int E = G::E;
+  constexpr int NQ = 2 + 4 * K;
+  const DevParams& p = *a.prm;
+  TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
+  double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
+
+  Xoshiro rng = lane_stream(a);
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i0 = lane * a.S;
+  const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
+
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+  for (uint64_t i = i0; i < i1; i++) {
+    double X[E], psi_tr, xi_tr;
+    sample_point<G>(rng, tv, X, psi_tr, xi_tr);
+    cplx out[K];
+    if (!integrand<G, K, MODE>(p, X, psi_tr, xi_tr, out)) {
+      acc[1] += 1.0;
+      continue;
+    }
+    acc[0] += 1.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      acc[2 + 4 * k] += out[k].re;
+      acc[3 + 4 * k] += out[k].re * out[k].re;
+      if constexpr (MODE == 2) {
+        acc[4 + 4 * k] += out[k].im;
+        acc[5 + 4 * k] += out[k].im * out[k].im;
+      }
+    }
+  }
+  /* wave shuffle tree, then LDS across waves, fixed order: deterministic */
+  const int wid = threadIdx.x >> 6, lid = threadIdx.x & 63, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const double v = wave_sum(acc[q]);
+    if (lid == 0) red[wid * NQ + q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += red[w * NQ + threadIdx.x];
+    a.partials[(size_t)blockIdx.x * NQ + threadIdx.x] = s;
+  }
+}
+
+/* Replay: evaluate the integrand on given points (one sample per lane).
+ * X: n x E, psixi: n x 2 -> out: n x K x 2, dropped: n. */
+template <class G, int K, int MODE>
+__device__ inline void replay_body(const DevParams* prm, uint64_t n, const double* Xin,
+                                   const double* psixi, double* out, uint8_t* dropped) {
+  constexpr int E = G::E;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double X[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) X[e] = Xin[i * E + e];
+  cplx r[K];
+  const bool ok = integrand<G, K, MODE>(*prm, X, psixi[2 * i], psixi[2 * i + 1], r);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    out[(i * K + k) * 2] = ok ? r[k].re : 0.0;
+    out[(i * K + k) * 2 + 1] = ok ? r[k].im : 0.0;
+  }
+  dropped[i] = ok ? 0 : 1;
+}
+
+/* Sampler only: lane l writes samples [l S, min((l+1) S, N)) as X, psixi. */
+template <class G, int TMODE>
+__device__ inline void sample_body(const Launch& a, double* Xout, double* pxout) {
+  extern __shared__ __align__(16) char lds[];
+  constexpr int E = G::E;
+  TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
+  Xoshiro rng = lane_stream(a);
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i0 = lane * a.S;
+  const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
+  for (uint64_t i = i0; i < i1; i++) {
+    double X[E], ps, xs;
+    sample_point<G>(rng, tv, X, ps, xs);
+#pragma unroll
+    for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
+    pxout[2 * i] = ps;
+    pxout[2 * i + 1] = xs;
+  }
+}
+
+}  // namespace ftr
+
+/* Instantiation macro used by the generated kernel units:
+ *   FTR_INSTANTIATE(name, Graph, K, MODE, TMODE, BLOCK)
+ * defines extern "C" kernels  name_int, name_rep, name_smp. */
+#define FTR_INSTANTIATE(NAME, G, K, MODE, TMODE, BLOCK)                                           \
+  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_int(ftr::Launch a) {                 \
+    ftr::integrate_body<G, K, MODE, TMODE>(a);                                                    \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(256)                                               \
+      NAME##_rep(const ftr::DevParams* prm, uint64_t n, const double* X, const double* px,        \
+                 double* out, uint8_t* dropped) {                                                 \
+    ftr::replay_body<G, K, MODE>(prm, n, X, px, out, dropped);                                    \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_smp(ftr::Launch a, double* X,        \
+                                                                   double* px) {                  \
+    ftr::sample_body<G, TMODE>(a, X, px);                                                         \
+  }

@@ -1,0 +1,45 @@
+"""Ahead-of-time kernel specialisation: compiles (hipRTC, no GPU needed) the
+kernels of every named workload into feyntrop_amd/kcache so that GPU runs of
+these problems load cached code objects instead of compiling.
+
+    python -m feyntrop_amd.precompile
+"""
+
+import concurrent.futures as cf
+import os
+import sys
+
+from feyntrop_amd import precompile, workloads
+
+
+def problems():
+    probs = dict(workloads.all_problems())
+    for k, v in workloads.scan_problems().items():
+        probs[k] = v
+    return probs
+
+
+def _one(item):
+    name, fn = item
+    try:
+        precompile(fn() if callable(fn) else fn)
+        return name, None
+    except Exception as e:  # report, keep going
+        return name, str(e)
+
+
+def main():
+    items = list(problems().items())
+    workers = min(8, os.cpu_count() or 1)
+    bad = 0
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        for name, err in ex.map(_one, items):
+            if err:
+                bad += 1
+                print(f"precompile {name}: {err}", file=sys.stderr)
+    print(f"precompiled {len(items) - bad}/{len(items)} problems")
+    return 1 if bad else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,141 @@
+/*
+ * feyntrop_amd.h -- C ABI of the MI355X tropical-sampling engine.
+ *
+ * Plain C types only (no torch, no STL): the CLI binary `feyntrop`, Python
+ * (ctypes) and any FFI bind these symbols.  Every call returns FTR_OK (0) or
+ * a negative status; ftr_last_error() gives the message of the last failure
+ * on that context.  All calls are synchronous with respect to the host unless
+ * stated otherwise; a context is re-entrant only from one host thread at a
+ * time.
+ *
+ * Reference interfaces replaced (michibo/feyntrop):
+ *   ftr_prepare     -- feyntrop.cpp:45-239: graph construction and validation
+ *                      (graph.hpp:169-214), scalar-product checks, regime
+ *                      detection (feyntrop.cpp:128-133) and
+ *                      generate_subgraph_table (tropical_sampling.hpp:284-458)
+ *   ftr_integrate   -- feynman_integral_estimate<Generator>(N, g, D, P, m2, K,
+ *                      lambda_eff, J, gen)  (feynman_integral.hpp:30-41,
+ *                      called at feyntrop.cpp:251) returning the raw moments
+ *                      that stats.hpp:73-134 holds
+ *   ftr_sample      -- get_random_psi_xi_tropical_sample
+ *                      (tropical_sampling.hpp:503-564) over a stream range
+ *   ftr_replay      -- the per-sample loop body of feynman_integral_estimate
+ *                      (feynman_integral.hpp:166-362) on given points
+ *   ftr_run_json    -- the feyntrop binary's stdin -> stdout JSON contract
+ *                      (feyntrop.cpp:20-281)
+ *   ftr_finalize    -- stats::avg/acc (stats.hpp:86-104)
+ */
+#ifndef FEYNTROP_AMD_H
+#define FEYNTROP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FTR_OK 0
+#define FTR_ERR_INVALID -1   /* invalid input (reference: std::invalid_argument) */
+#define FTR_ERR_DOMAIN -2    /* mathematically unsupported input (std::domain_error) */
+#define FTR_ERR_DEVICE -3    /* HIP runtime failure */
+#define FTR_ERR_COMPILE -4   /* kernel specialisation failed */
+#define FTR_ERR_LIMIT -5     /* outside this engine's size limits */
+
+typedef struct ftr_ctx ftr_ctx;
+typedef struct ftr_problem ftr_problem;
+
+/* One Feynman-integral problem, as in the low-level JSON input. */
+typedef struct {
+  int E;                  /* number of edges */
+  int V;                  /* number of rows/cols of P (must equal the vertex count) */
+  const int32_t* edges;   /* 2*E vertex ids, as given */
+  const double* nu;       /* E edge weights */
+  const double* P;        /* V*V scalar products, row-major */
+  const double* m2;       /* E squared masses */
+  double D;               /* dimension */
+  int K;                  /* num_eps_terms */
+  double lambda;          /* deformation parameter */
+} ftr_input;
+
+/* Preprocessing results (J-table scalars and regime flags). */
+typedef struct {
+  int E, V, L, K;
+  double W, IGtr;
+  int euclidean, pseudo_euclidean, generic, gp_property;
+  double lambda_eff;      /* lambda if the deformation is active, else 0 */
+  int path;               /* 0 real (psi, phi), 1 real (psi only), 2 deformed complex */
+  double seconds_preprocessing;
+} ftr_info;
+
+/* Raw moments of one integration: per order k and part q (0 re, 1 im):
+ * n[k][q], s1[k][q], s2[k][q]; plus dropped samples and device timing. */
+#define FTR_MAX_K 16
+typedef struct {
+  int K;
+  double n[FTR_MAX_K][2];
+  double s1[FTR_MAX_K][2];
+  double s2[FTR_MAX_K][2];
+  uint64_t dropped;
+  double kernel_ms;       /* HIP-event time of the sampling kernel(s) */
+  double seconds;         /* wall time of the call */
+  uint64_t lanes;         /* lanes launched */
+  uint64_t samples_per_lane;
+} ftr_moments;
+
+/* Launch options (0 = automatic). */
+typedef struct {
+  int block;              /* threads per block: 256 or 1024 */
+  int table_mode;         /* -1 auto, 0 LDS, 1 w in LDS + rest global, 2 global */
+  uint64_t lanes;         /* total lanes; 0 = fill the device */
+} ftr_opts;
+
+int ftr_create(int device, ftr_ctx** ctx);
+void ftr_destroy(ftr_ctx* ctx);
+const char* ftr_last_error(const ftr_ctx* ctx);
+
+/* Validate + preprocess + specialise kernels.  *out owned by the caller. */
+int ftr_prepare(ftr_ctx* ctx, const ftr_input* in, ftr_problem** out);
+void ftr_problem_free(ftr_problem* p);
+int ftr_problem_info(const ftr_problem* p, ftr_info* info);
+/* J-table: arrays of 2^E entries (Jr, r, mm, L). */
+int ftr_jtable(const ftr_problem* p, double* Jr, double* r, uint8_t* mm, uint8_t* L);
+
+/* Integrate N samples of the single xoshiro256+ stream seeded with `seed`:
+ * sample i uses stream positions [i(2E-1), (i+1)(2E-1)), so the multiset of
+ * samples equals the reference run with one OpenMP thread. */
+int ftr_integrate(ftr_ctx* ctx, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts,
+                  ftr_moments* out);
+
+/* Final estimates from moments like stats.hpp:86-104 (real path: the
+ * imaginary parts are exactly 0, as with the reference's zero updates).
+ * res: K x 4 = (re, dre, im, dim). */
+int ftr_finalize(const ftr_problem* p, const ftr_moments* m, double* res);
+
+/* Device-pointer entry points for parity work (caller-owned HBM buffers):
+ * samples [first, first+n) of the stream seeded with seed -> X (n*E), psixi (n*2). */
+int ftr_sample(ftr_ctx* ctx, ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
+               double* X_dev, double* psixi_dev);
+/* Integrand on given points: X (n*E), psixi (n*2) -> out (n*K*2), dropped (n). */
+int ftr_replay(ftr_ctx* ctx, ftr_problem* p, uint64_t n, const double* X_dev,
+               const double* psixi_dev, double* out_dev, uint8_t* dropped_dev);
+
+/* Full CLI contract: JSON text in -> JSON text out (caller frees with
+ * ftr_free_string); log lines as the reference writes to std::clog. */
+int ftr_run_json(ftr_ctx* ctx, const char* json_in, char** json_out, char** log_out);
+void ftr_free_string(char* s);
+
+/* Batched scan: n problems, one launch of a persistent kernel per batch. */
+int ftr_integrate_batch(ftr_ctx* ctx, int n, ftr_problem* const* probs, const uint64_t* N,
+                        const uint64_t* seeds, ftr_moments* out);
+
+/* Kernel code-object cache directory (default: the package's kcache/). */
+int ftr_set_cache_dir(const char* dir);
+
+/* Compile (hipRTC) the kernels for a problem into the cache without a GPU. */
+int ftr_precompile(const ftr_input* in, char* err, int errlen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

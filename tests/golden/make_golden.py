@@ -1,0 +1,110 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Run in the build container, where the reference checkout exists and
+`make -C oracle ref` has built oracle/_ref/libref_harness.so (the reference
+sources compiled where they lie, by oracle/Makefile):
+
+    python tests/golden/make_golden.py
+
+Outputs (all data, no reference source):
+  rng.json        raw xoshiro256+ outputs and the states after the fixed
+                  2^128 / 2^192 jumps (random.hpp:86-199)
+  problems.json   per named problem: regime flags, W, L, IGtr, the J-table
+                  (full for E <= 10, digests for larger E), the first sample
+                  points of stream seed 0 (hex doubles), the first per-sample
+                  integrand values, and one-thread estimates (moments) at a
+                  modest N
+  fixtures.json   the reference test suite's expected results
+                  (tests/*.txt), the README output of the K4 wheel with
+                  masses [1..6] and the tutorial notebook's printed result
+"""
+
+import ast
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from feyntrop_amd import workloads as W  # noqa: E402
+from oracle_lib import Ref, REF_LIB  # noqa: E402
+
+REF_DIR = os.environ.get("FEYNTROP_REFERENCE", "/root/reference")
+N_SAMPLES = 40
+N_VALUES = 200
+N_EST = {"default": 50000, "cubic5": 20000, "triple_ladder": 20000}
+
+
+def hexs(a):
+    return [float(x).hex() for x in np.asarray(a, dtype=np.float64).reshape(-1)]
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    import ctypes as C
+    if not os.path.exists(REF_LIB):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    R = Ref()
+    lib = R.lib
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+    lib.ref_xoshiro.argtypes = [C.c_uint64, C.c_uint64, u64p, u64p, u64p]
+
+    rng = {}
+    for seed in (0, 1, 12345, 2**63 + 5):
+        out = np.zeros(64, np.uint64)
+        j = np.zeros(4, np.uint64)
+        lj = np.zeros(4, np.uint64)
+        lib.ref_xoshiro(seed, 64, out, j, lj)
+        rng[str(seed)] = {"next": [int(x) for x in out], "jump": [int(x) for x in j],
+                          "long_jump": [int(x) for x in lj]}
+    json.dump(rng, open(os.path.join(HERE, "rng.json"), "w"), indent=1)
+
+    probs = {}
+    for name, fn in W.all_problems().items():
+        p = fn()
+        rp = R.problem(p)
+        d = {"info": [float(x) for x in rp.info[:10]]}
+        Jr, r, mm, Lt = rp.jtable()
+        if rp.E <= 10:
+            d["jtable"] = {"Jr": hexs(Jr), "r": hexs(r), "mm": mm.tolist(), "L": Lt.tolist()}
+        d["jtable_digest"] = {"Jr": digest(Jr), "r": digest(r), "mm": digest(mm), "L": digest(Lt)}
+        st = np.zeros(4, np.uint64)
+        from oracle_lib import Oracle  # seeding only (splitmix64, checked against rng.json)
+        st = Oracle().seed(0)
+        X, px, _ = rp.sample(st, N_SAMPLES)
+        d["samples"] = {"X": hexs(X), "psixi": hexs(px)}
+        vals, drop, _ = rp.values(Oracle().seed(0), N_VALUES)
+        d["values"] = {"out": hexs(vals), "dropped": drop.tolist()}
+        nest = N_EST.get(name, N_EST["default"])
+        m = rp.estimate(nest, seed=0, threads=1)
+        d["estimate"] = {"N": nest, "moments": hexs(m)}
+        probs[name] = d
+        print(name, "IGtr", rp.IGtr, "flags", rp.info[5:9])
+    json.dump(probs, open(os.path.join(HERE, "problems.json"), "w"))
+
+    fx = {}
+    for name in W.FIXTURES:
+        with open(os.path.join(REF_DIR, "tests", name + ".txt")) as f:
+            fx[name] = ast.literal_eval(f.read())
+    fx["_readme_wheel_masses_1_6"] = {
+        "IGtr": 84.0,
+        "integral": [[[7.215238614660525, 0.00203586844683068], [0.0, 0.0]],
+                     [[-57.629482716637696, 0.018239280410844466], [0.0, 0.0]],
+                     [[240.79344300578586, 0.09697082078903732], [0.0, 0.0]]]}
+    fx["_tutorial_notebook"] = [[(-46.56, 0.13), (87.18, 0.12)], [(-274.39, 0.55), (111.34, 0.55)],
+                                [(-435.06, 1.30), (-174.30, 1.33)], [(-191.81, 2.15), (-494.58, 2.14)],
+                                [(219.08, 2.69), (-431.85, 2.67)]]
+    json.dump(fx, open(os.path.join(HERE, "fixtures.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

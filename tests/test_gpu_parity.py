@@ -1,0 +1,228 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+golden vectors of the compiled reference.
+
+Bars (written in each test):
+  sampler        cut order identical, X / psi_tr / xi_tr within 1e-12 relative
+                 (the only difference allowed is the last-bit rounding of pow)
+  per sample     on samples where the reference algorithm is itself accurate
+                 (strict double vs long-double oracle within 1e-13), GPU vs
+                 strict oracle within 1e-12 relative except for a small
+                 fraction; over all samples the GPU's error against the
+                 long-double oracle is no larger than the reference
+                 algorithm's own (quantiles)
+  estimates      the same samples as the reference run with one thread:
+                 sample count exact, every mean within 1e-6 of its standard
+                 error of the reference one-thread estimate
+  statistics     every eps coefficient of the 7 reference fixtures within
+                 3 combined standard errors; K4 wheel eps^0 = 6 zeta(3)
+"""
+
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from feyntrop_amd import workloads as W
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ["wheel_k4", "readme_wheel", "double_box", "cubic5", "triple_ladder"] + list(W.FIXTURES)
+
+
+def _hex(a):
+    return np.array([float.fromhex(x) for x in a])
+
+
+def _dev(n):
+    return torch.zeros(int(n), dtype=torch.float64, device="cuda")
+
+
+def _crel(a, b):
+    za = a[..., 0] + 1j * a[..., 1]
+    zb = b[..., 0] + 1j * b[..., 1]
+    return np.abs(za - zb) / np.maximum(np.abs(zb), 1e-300)
+
+
+def gpu_sample(engine, p, first, n, seed=0):
+    X = _dev(n * p.E)
+    px = _dev(2 * n)
+    torch.cuda.synchronize()
+    engine.sample(p, first, n, seed, X.data_ptr(), px.data_ptr())
+    return X.cpu().numpy().reshape(n, p.E), px.cpu().numpy().reshape(n, 2)
+
+
+def gpu_replay(engine, p, X, px):
+    n = X.shape[0]
+    Xd = torch.from_numpy(np.ascontiguousarray(X.reshape(-1))).cuda()
+    pd = torch.from_numpy(np.ascontiguousarray(px.reshape(-1))).cuda()
+    out = _dev(n * p.K * 2)
+    drop = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    engine.replay(p, n, Xd.data_ptr(), pd.data_ptr(), out.data_ptr(), drop.data_ptr())
+    return out.cpu().numpy().reshape(n, p.K, 2), drop.cpu().numpy()
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_sampler_matches_reference_stream(engine, golden_problems, name):
+    """Samples 0..39 of seed 0 against the compiled reference (golden), and
+    a run further down the stream (first = 777) against the oracle."""
+    print(f"[sampler] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    g = golden_problems[name]["samples"]
+    Xr = _hex(g["X"]).reshape(-1, p.E)
+    pr = _hex(g["psixi"]).reshape(-1, 2)
+    X, px = gpu_sample(engine, p, 0, Xr.shape[0])
+    assert np.array_equal(np.argsort(-X, axis=1, kind="stable"), np.argsort(-Xr, axis=1, kind="stable"))
+    assert np.max(np.abs(X - Xr) / np.abs(Xr)) <= 1e-12
+    assert np.max(np.abs(px - pr) / np.maximum(np.abs(pr), 1e-300)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["wheel_k4", "double_box", "cubic5", "2L_5pt_box_pent"])
+def test_sampler_jump_ahead(engine, oracle, name):
+    """Stream offset by jump-ahead: samples [777, 777+20000) of seed 5."""
+    print(f"[jump] {name}", flush=True)
+    prob = W.all_problems()[name]()
+    p = engine.prepare(prob)
+    op = oracle.problem(prob)
+    first, n = 777, 20000
+    st = oracle.seed(5)
+    oracle.advance(st, first * (2 * p.E - 1))
+    Xo, po, _ = op.sample(st, n)
+    X, px = gpu_sample(engine, p, first, n, seed=5)
+    assert np.array_equal(np.argsort(-X, axis=1, kind="stable"), np.argsort(-Xo, axis=1, kind="stable"))
+    assert np.max(np.abs(X - Xo) / np.abs(Xo)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_replay_on_reference_points(engine, golden_problems, name):
+    """GPU integrand on the reference's own first 40 sample points vs the
+    reference's per-sample values (from its own N=1 estimates)."""
+    print(f"[replay-golden] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    g = golden_problems[name]
+    X = _hex(g["samples"]["X"]).reshape(-1, p.E)
+    px = _hex(g["samples"]["psixi"]).reshape(-1, 2)
+    ref = _hex(g["values"]["out"]).reshape(-1, p.K, 2)[: X.shape[0]]
+    out, drop = gpu_replay(engine, p, X, px)
+    assert not drop.any()
+    r = _crel(out, ref)
+    assert np.median(r) <= 1e-13
+    assert np.mean(r > 1e-12) <= 0.05
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_replay_parity_mask(engine, oracle, oracle_ld, name):
+    """100k oracle points: accuracy against the long-double oracle matches
+    the reference algorithm's own, and on the well-conditioned mask the GPU
+    equals the strict oracle to 1e-12 (allowing a 0.5% tail, max 1e-8)."""
+    print(f"[replay-mask] {name}", flush=True)
+    prob = W.all_problems()[name]()
+    p = engine.prepare(prob)
+    op, ol = oracle.problem(prob), oracle_ld.problem(prob)
+    n = 100000
+    X, px, _ = op.sample(oracle.seed(11), n)
+    ref, rdrop, _ = op.replay(X, px)
+    tru, _, _ = ol.replay(X, px)
+    out, drop = gpu_replay(engine, p, X, px)
+    assert np.array_equal(drop.astype(bool), rdrop.astype(bool))
+    e_gpu = _crel(out, tru)[:, 0]
+    e_ref = _crel(ref, tru)[:, 0]
+    for q in (0.5, 0.9, 0.99, 0.999):
+        assert np.quantile(e_gpu, q) <= 2.0 * np.quantile(e_ref, q) + 2e-15, q
+    mask = e_ref <= 1e-13
+    d = _crel(out, ref)[:, 0][mask]
+    print(f"   mask {mask.mean():.4f} median {np.median(d):.2e} frac>1e-12 {np.mean(d > 1e-12):.5f} max {d.max():.2e}")
+    assert np.median(d) <= 1e-14
+    assert np.mean(d > 1e-12) <= 0.005
+    assert d.max() <= 1e-8
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_estimate_equals_reference_one_thread(engine, golden_problems, name):
+    """integrate(N, seed 0) draws the reference's one-thread samples."""
+    print(f"[estimate] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    g = golden_problems[name]["estimate"]
+    N = g["N"]
+    ref = _hex(g["moments"]).reshape(p.K, 2, 3)
+    m = engine.integrate(p, N, 0)
+    n, s1, s2 = m.arrays()
+    assert m.dropped == 0
+    for k in range(p.K):
+        assert n[k, 0] == ref[k, 0, 0]
+        parts = (0, 1) if p.info.path == 2 else (0,)
+        for q in parts:
+            rn, rs1, rs2 = ref[k, q]
+            sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+            assert abs(s1[k, q] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn), (k, q)
+            assert abs(s2[k, q] - rs2) <= 1e-9 * abs(rs2) + 1e-300, (k, q)
+
+
+@pytest.mark.parametrize("name", ["wheel_k4", "double_box", "2L_3pt_tutorial"])
+def test_estimate_independent_of_launch_geometry(engine, name):
+    """The multiset of samples depends only on (N, seed): two lane counts
+    give identical counts and sums equal up to summation order."""
+    print(f"[geometry] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    N = 3_000_000
+    a = engine.integrate(p, N, 3, lanes=4096)
+    b = engine.integrate(p, N, 3, lanes=0)
+    assert a.samples_per_lane != b.samples_per_lane
+    na, s1a, s2a = a.arrays()
+    nb, s1b, s2b = b.arrays()
+    assert np.array_equal(na, nb)
+    assert np.allclose(s1a, s1b, rtol=1e-11, atol=1e-11 * np.abs(s1a).max())
+    assert np.allclose(s2a, s2b, rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("name", list(W.FIXTURES))
+def test_fixture_within_3_sigma(engine, golden_fixtures, name):
+    """Every eps coefficient (re, im) of the reference test suite at
+    N = 1e7 within 3 combined standard errors."""
+    print(f"[fixture] {name}", flush=True)
+    prob = W.FIXTURES[name]()
+    p = engine.prepare(prob)
+    m = engine.integrate(p, prob["N"], 0)
+    res = p.finalize(m)
+    gold = golden_fixtures[name]
+    for k in range(p.K):
+        for q in range(2):
+            a, da = res[k, q]
+            b, db = gold[k][q]
+            assert abs(a - b) <= 3.0 * np.hypot(da, db) + 1e-12, (k, q, a, da, b, db)
+
+
+def test_wheel_period_6zeta3(engine):
+    """eps^0 of the K4 wheel is its period 6 zeta(3) (W = 0: mass independent)."""
+    prob = W.wheel_k4()
+    p = engine.prepare(prob)
+    m = engine.integrate(p, 10**7, 0)
+    r = p.finalize(m)
+    z3 = 1.2020569031595942853997381615114
+    assert abs(r[0, 0, 0] - 6 * z3) <= 3 * r[0, 0, 1]
+    assert r[0, 1, 0] == 0.0 and r[0, 1, 1] == 0.0
+
+
+def test_cli_readme_wheel(golden_fixtures):
+    """The feyntrop binary on the shipped low-level input (masses [1..6]):
+    output keys/format and the README result within 3 sigma."""
+    import feyntrop_amd
+    prob = W.readme_wheel()
+    out = subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps(prob), capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    line = out.stdout.strip()
+    assert line.startswith('{"IGtr":84.0,"integral":[[[')
+    d = json.loads(line)
+    assert list(d.keys()) == ["IGtr", "integral", "seconds preprocessing", "seconds sampling"]
+    gold = golden_fixtures["_readme_wheel_masses_1_6"]["integral"]
+    for k in range(3):
+        a, da = d["integral"][k][0]
+        b, db = gold[k][0]
+        assert abs(a - b) <= 3 * np.hypot(da, db)
+        assert d["integral"][k][1] == [0.0, 0.0]
+    assert "(Effective) kinematic regime: Euclidean (generic)." in out.stderr
