@@ -1,0 +1,147 @@
+"""Benchmark: plain tropical sampling throughput on one MI355X.
+
+Workload (N=1): the 2-loop 4-point double box in the deformed Minkowski regime
+(7 edges, 6 vertices, D = 6, m^2 = 1, s, t from seed 0, lambda = 0.5, two
+eps orders), N = 1e8 samples per step, FP64 complex-LU path.  One step is one
+ftr_integrate call: 1e8 samples drawn, evaluated and reduced on the GPU.
+
+Reported: samples/s (value), time to 1e-3 relative error of the eps^0
+coefficient extrapolated from the measured error, the FP64 roofline of the
+sampling kernel (algorithmic flops per sample x samples per launch / mean
+HIP-event duration of the launch), and the reference CPU path (compiled from
+its sources, OpenMP on the host cores) on a bounded sample of the same
+workload as the baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload double_box]
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+
+
+def flops_per_sample(info):
+    """Algorithmic FP64 flops of one sample (+ - x /, FMA = 2; transcendentals
+    excluded), fixed by (E, V, K, path):
+      sampler   E(E+1) + 4E
+      real      n^3/3 (LDL^T) + 2n^3 (n-RHS solve) + n^2 + V^2 + 12E + 3K
+      complex   n^3/3 + 2n^3 (inverse) + 4n^3 (LPGPLT) + 5.5E^2 (derivatives)
+                + 8/3 n^3 (complex LU) + 8n^3 (complex n-RHS solve)
+                + 8/3 E^3 (complex Jacobian LU) + 2E^2 + 40E + 8K
+    with n = V - 1."""
+    E, V, K = info.E, info.V, info.K
+    n = V - 1
+    f = E * (E + 1) + 4 * E
+    if info.path == 2:
+        f += (n ** 3 / 3 + 2 * n ** 3 + 4 * n ** 3 + 5.5 * E ** 2 + 8 / 3 * n ** 3 + 8 * n ** 3
+              + 8 / 3 * E ** 3 + 2 * E ** 2 + 40 * E + 8 * K)
+    else:
+        f += n ** 3 / 3 + 2 * n ** 3 + n ** 2 + V ** 2 + 12 * E + 3 * K
+    return f
+
+
+def cpu_baseline(prob, n_samples, threads):
+    """The reference binary (oracle/_ref/feyntrop_ref, built from the reference
+    sources with its own flags) on a bounded sample of the workload."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "feyntrop_ref")
+    kind = "reference"
+    if not os.path.exists(ref):
+        return None
+    q = dict(prob)
+    q["N"] = int(n_samples)
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    t0 = time.time()
+    out = subprocess.run([ref], input=json.dumps(q), capture_output=True, text=True, env=env, timeout=600)
+    wall = time.time() - t0
+    if out.returncode != 0:
+        return None
+    d = json.loads(out.stdout)
+    secs = d["seconds sampling"]
+    return {"value": q["N"] / secs, "unit": "samples/s", "cores": threads, "kind": kind,
+            "sample": f"same workload, N={q['N']:.0e}, seed 0, OMP_NUM_THREADS={threads} "
+                      f"('seconds sampling' {secs:.2f} s, wall {wall:.2f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="double_box")
+    ap.add_argument("--N", type=float, default=0)
+    ap.add_argument("--cpu-samples", type=float, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import feyntrop_amd
+    from feyntrop_amd import workloads as W
+
+    prob = W.all_problems()[args.workload]()
+    if args.N:
+        prob["N"] = int(args.N)
+    N = int(prob["N"])
+    eng = feyntrop_amd.Engine(0)
+    p = eng.prepare(prob)
+    for i in range(args.warmup):
+        eng.integrate(p, N, i + 1)
+    kms, res, seeds = [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        m = eng.integrate(p, N, 1000 + i)
+        kms.append(m.kernel_ms)
+        res.append(m)
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    value = N * args.steps / wall
+
+    # accuracy of eps^0 from the combined moments of all timed steps
+    rr = [p.finalize(m) for m in res]
+    I0 = np.mean([r[0, 0, 0] for r in rr]) + 1j * np.mean([r[0, 1, 0] for r in rr])
+    d0 = np.mean([np.hypot(r[0, 0, 1], r[0, 1, 1]) for r in rr])
+    rel_at_N = d0 / abs(I0)
+    t_1e3 = (rel_at_N / 1e-3) ** 2 * N / value
+
+    fps = flops_per_sample(p.info)
+    mean_kms = float(np.mean(kms))
+    achieved = fps * N / (mean_kms * 1e-3) / 1e12
+    line = {
+        "metric": "samples/s and time-to-1e-3 rel. err, plain vs learned proposal, 1 MI355X",
+        "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (kinematics from seed 0)",
+        "config": {"workload": f"{args.workload}: E={p.E} V={p.V} K={p.K} path="
+                               f"{['real', 'real-psi', 'deformed-complex'][p.info.path]}",
+                   "N_per_step": N, "lambda": prob["lambda"], "dimension": prob["dimension"]},
+        "time_to_1e-3_s": t_1e3,
+        "rel_err_eps0_at_N": rel_at_N,
+        "eps0": [I0.real, I0.imag],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "ftrk_int", "flops_per_sample": fps, "kernel_ms_mean": mean_kms},
+        "learned": None,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ns = args.cpu_samples or 2.5e6 * threads
+        line["cpu_baseline"] = cpu_baseline(prob, ns, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -995,6 +995,13 @@ void or_xoshiro_jumppoly_pow2(int e, uint64_t* p4) {
   for (int i = 0; i < 4; i++) p4[i] = b.w[i];
 }
 
+void or_xoshiro_jump_pow2(int e, uint64_t* st) {
+  compute_charpoly();
+  Poly4 b; memset(&b, 0, sizeof b); b.w[0] = 2;
+  for (int i = 0; i < e; i++) b = poly_mulmod(b, b);
+  poly_apply(b, st);
+}
+
 double or_fp_sum(const double* v, uint64_t n) {
   Summer s;
   for (uint64_t i = 0; i < n; i++) s.add(v[i]);

@@ -90,6 +90,9 @@ void or_xoshiro_charpoly(uint64_t* poly5);
 /* x^(2^e) mod charpoly, 4 words (the jump polynomial for distance 2^e). */
 void or_xoshiro_jumppoly_pow2(int e, uint64_t* poly4);
 
+/* Advance st by 2^e steps (e may exceed 127). */
+void or_xoshiro_jump_pow2(int e, uint64_t* st);
+
 /* fp_summer restatement (stats.hpp:13-57) on a stream, for tests. */
 double or_fp_sum(const double* v, uint64_t n);
 

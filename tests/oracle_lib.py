@@ -104,6 +104,7 @@ class Oracle(_Lib):
         f("xoshiro_advance").argtypes = [_u64p, C.c_uint64, C.c_uint64]
         f("xoshiro_charpoly").argtypes = [_u64p]
         f("xoshiro_jumppoly_pow2").argtypes = [C.c_int, _u64p]
+        f("xoshiro_jump_pow2").argtypes = [C.c_int, _u64p]
         f("fp_sum").restype = C.c_double
         f("fp_sum").argtypes = [_f64p, C.c_uint64]
 

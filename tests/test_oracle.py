@@ -1,0 +1,240 @@
+"""Pins of the CPU oracle (CPU only).
+
+Each check is chosen so that a plausible mistake fails it:
+  - RNG: raw outputs of the compiled reference; jump polynomials equal the
+    generator's published 2^128 / 2^192 jump constants; arbitrary advance
+    equals stepping by brute force.
+  - J-table, flags, W, L, IGtr: bitwise against the compiled reference.
+  - sampler: bitwise against the compiled reference on the same stream.
+  - integrand: per sample against the compiled reference; psi and the F/U
+    ratio against matrix-tree (spanning-tree / spanning-2-forest) brute force.
+  - estimates: the reference's one-thread moments; the reference test suite
+    within 3 sigma; the K4 wheel period 6 zeta(3); lambda independence of the
+    deformed integral (Cauchy's theorem).
+  - fp_summer: exact rational sums.
+"""
+
+import itertools
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from feyntrop_amd import workloads as W
+
+NAMES = list(W.all_problems())
+
+# xoshiro256+ jump constants published with the generator (x^(2^128) and
+# x^(2^192) modulo its characteristic polynomial).
+JUMP = [0x180ec6d33cfd0aba, 0xd5a61266f0c9392c, 0xa9582618e03fc9aa, 0x39abdc4529b1661c]
+LONG_JUMP = [0x76e15d3efefdcbbf, 0xc5004e441c522fb3, 0x77710069854ee241, 0x39109bb02acbe635]
+
+
+def _hex(a):
+    return np.array([float.fromhex(x) for x in a])
+
+
+def test_rng_outputs(oracle, golden_rng):
+    for seed, g in golden_rng.items():
+        st = oracle.seed(int(seed))
+        assert [int(x) for x in oracle.next(st, 64)] == g["next"]
+
+
+def test_jump_polynomials_are_published_constants(oracle):
+    q = np.zeros(4, np.uint64)
+    oracle.fn("xoshiro_jumppoly_pow2")(128, q)
+    assert [int(x) for x in q] == JUMP
+    oracle.fn("xoshiro_jumppoly_pow2")(192, q)
+    assert [int(x) for x in q] == LONG_JUMP
+
+
+def test_jump_states(oracle, golden_rng):
+    for seed, g in golden_rng.items():
+        st = oracle.seed(int(seed))
+        oracle.fn("xoshiro_jump_pow2")(128, st)
+        assert [int(x) for x in st] == g["jump"]
+        st = oracle.seed(int(seed))
+        oracle.fn("xoshiro_jump_pow2")(192, st)
+        assert [int(x) for x in st] == g["long_jump"]
+        # and the 128-bit advance agrees with the polynomial jump below 2^128
+        a, b = oracle.seed(int(seed)), oracle.seed(int(seed))
+        oracle.advance(a, (1 << 127) + 12345)
+        oracle.fn("xoshiro_jump_pow2")(127, b)
+        oracle.advance(b, 12345)
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("d", [1, 2, 13, 255, 256, 257, 1000, 40961])
+def test_advance_equals_stepping(oracle, d):
+    a = oracle.seed(99)
+    b = oracle.seed(99)
+    oracle.advance(a, d)
+    oracle.next(b, d)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_jtable_and_flags(oracle, golden_problems, name):
+    op = oracle.problem(W.all_problems()[name]())
+    g = golden_problems[name]
+    assert np.array_equal(op.info[:10], np.array(g["info"]))
+    Jr, r, mm, L = op.jtable()
+    if "jtable" in g:
+        assert np.array_equal(Jr, _hex(g["jtable"]["Jr"]))
+        assert np.array_equal(r, _hex(g["jtable"]["r"]))
+        assert mm.tolist() == g["jtable"]["mm"]
+        assert L.tolist() == g["jtable"]["L"]
+    import hashlib
+    dig = {k: hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest()
+           for k, v in (("Jr", Jr), ("r", r), ("mm", mm), ("L", L))}
+    assert dig == g["jtable_digest"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_samples_bitwise(oracle, golden_problems, name):
+    op = oracle.problem(W.all_problems()[name]())
+    g = golden_problems[name]["samples"]
+    X, px, _ = op.sample(oracle.seed(0), len(g["psixi"]) // 2)
+    assert np.array_equal(X.reshape(-1), _hex(g["X"]))
+    assert np.array_equal(px.reshape(-1), _hex(g["psixi"]))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_values_vs_reference(oracle, golden_problems, name):
+    op = oracle.problem(W.all_problems()[name]())
+    g = golden_problems[name]
+    n = len(g["values"]["dropped"])
+    X, px, _ = op.sample(oracle.seed(0), n)
+    out, drop, _ = op.replay(X, px)
+    ref = _hex(g["values"]["out"]).reshape(n, op.K, 2)
+    assert drop.tolist() == g["values"]["dropped"]
+    z = np.abs((out[..., 0] - ref[..., 0]) + 1j * (out[..., 1] - ref[..., 1]))
+    rel = z / np.maximum(np.abs(ref[..., 0] + 1j * ref[..., 1]), 1e-300)
+    if op.info[11] == 0:   # real path: 1e-12 on every sample
+        assert rel.max() <= 1e-12
+    else:                  # deformed path: the reference's own flag-to-flag spread
+        assert np.median(rel) <= 1e-14
+        assert np.mean(rel[:, 0] > 1e-12) <= 0.05
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_estimate_vs_reference_one_thread(oracle, golden_problems, name):
+    op = oracle.problem(W.all_problems()[name]())
+    g = golden_problems[name]["estimate"]
+    ref = _hex(g["moments"]).reshape(op.K, 2, 3)
+    m, drops = op.estimate(g["N"], 0, chunks=4, threads=4)
+    assert drops == 0
+    for k in range(op.K):
+        for q in ((0, 1) if op.info[11] else (0,)):
+            rn, rs1, rs2 = ref[k, q]
+            assert m[k, q, 0] == rn
+            sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+            assert abs(m[k, q, 1] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn)
+
+
+# ------------------------------------------------- matrix-tree brute force --
+
+def _spanning_forests(V, edges, ncomp):
+    """Edge subsets that form spanning forests with ncomp trees; yields
+    (subset, component labels)."""
+    E = len(edges)
+    for sub in itertools.combinations(range(E), V - ncomp):
+        parent = list(range(V))
+
+        def find(a):
+            while parent[a] != a:
+                parent[a] = parent[parent[a]]
+                a = parent[a]
+            return a
+        ok = True
+        for e in sub:
+            a, b = find(edges[e][0]), find(edges[e][1])
+            if a == b:
+                ok = False
+                break
+            parent[a] = b
+        if ok:
+            yield sub, [find(v) for v in range(V)]
+
+
+def _psi_phi(prob, x):
+    """Kirchhoff polynomial psi(x) = sum_T prod_{e not in T} x_e and
+    phi(x) = -sum_{2-forests (T1,T2)} p(T1)^2 prod_{e not in F} x_e, with
+    p(T1)^2 = sum_{u,v in T1} P_uv; the F/U ratio is phi/psi + sum m^2 x."""
+    edges = [tuple(e[0]) for e in prob["graph"]]
+    V = max(max(e) for e in edges) + 1
+    P = np.array(prob["scalarproducts"], float)
+    psi = 0.0
+    for sub, _ in _spanning_forests(V, edges, 1):
+        psi += np.prod([x[e] for e in range(len(edges)) if e not in sub])
+    phi = 0.0
+    for sub, lab in _spanning_forests(V, edges, 2):
+        comp = [v for v in range(V) if lab[v] == lab[0]]
+        p2 = sum(P[u, v] for u in comp for v in comp)
+        phi += -p2 * np.prod([x[e] for e in range(len(edges)) if e not in sub])
+    return psi, phi
+
+
+@pytest.mark.parametrize("name", ["wheel_k4", "1L_3pt_eucl", "readme_wheel"])
+def test_integrand_matrix_tree(oracle, name):
+    """Real path: R0 = IGtr (psi(X)/psi_tr)^(-D/2) (xi(X)/xi_tr)^(-W) and
+    R1/R0 = log psi' - L log xi' (primes: after the projective rescale), with
+    psi, phi from spanning trees / 2-forests."""
+    prob = W.all_problems()[name]()
+    op = oracle.problem(prob)
+    X, px, _ = op.sample(oracle.seed(3), 300)
+    out, drop, _ = op.replay(X, px)
+    D, Wd, L, IG = prob["dimension"], op.W, op.L_loops, op.IGtr
+    m2 = np.array(prob["masses_sqr"], float)
+    for i in range(X.shape[0]):
+        psi, phi = _psi_phi(prob, X[i])
+        xi = phi / psi + m2 @ X[i]          # the F/U ratio (degree 1)
+        tgt = px[i, 0] ** (-D / 2) * (px[i, 0] / px[i, 1]) ** Wd
+        s = tgt ** (1 / (D / 2 * L + Wd))
+        psi_s, xi_s = psi * s ** L, xi * s
+        R0 = IG * psi_s ** (-D / 2) * xi_s ** (-Wd)
+        assert abs(out[i, 0, 0] - R0) <= 1e-11 * abs(R0)
+        if op.K > 1:
+            ef = np.log(psi_s) - L * np.log(xi_s)
+            assert abs(out[i, 1, 0] - R0 * ef) <= 1e-10 * (abs(R0 * ef) + abs(R0))
+
+
+def test_fp_summer_exact(oracle):
+    rng = np.random.default_rng(1)
+    for scale in (1.0, 1e10):
+        v = rng.standard_normal(100000) * scale * np.exp(rng.uniform(-20, 20, 100000))
+        exact = sum(Fraction(float(x)) for x in v)
+        got = oracle.fn("fp_sum")(np.ascontiguousarray(v), len(v))
+        assert abs(Fraction(got) - exact) <= Fraction(1e-15) * sum(Fraction(abs(float(x))) for x in v)
+
+
+@pytest.mark.parametrize("name", list(W.FIXTURES))
+def test_reference_suite_3sigma(oracle, golden_fixtures, name):
+    """The oracle's single-stream estimate at N = 1e6 against the reference
+    test suite results (tests/*.txt) within 3 combined standard errors."""
+    prob = W.FIXTURES[name](N=10**6)
+    op = oracle.problem(prob)
+    m, drops = op.estimate(prob["N"], 0, chunks=64, threads=8)
+    r = op.finalize(m)
+    g = golden_fixtures[name]
+    for k in range(op.K):
+        for q in range(2):
+            a, da = r[k, q]
+            b, db = g[k][q]
+            assert abs(a - b) <= 3 * np.hypot(da, db) + 1e-12
+
+
+def test_lambda_independence(oracle):
+    """The deformed integral does not depend on lambda (Cauchy's theorem);
+    a wrong deformation Jacobian or phase would make it depend on lambda."""
+    res = []
+    for lam in (3.0, 7.6):
+        prob = W.fixture_2L_3pt_tutorial(N=400000)
+        prob["lambda"] = lam
+        op = oracle.problem(prob)
+        m, _ = op.estimate(prob["N"], 1, chunks=64, threads=8)
+        res.append(op.finalize(m))
+    a, b = res
+    for k in range(2):
+        for q in range(2):
+            assert abs(a[k, q, 0] - b[k, q, 0]) <= 4 * np.hypot(a[k, q, 1], b[k, q, 1])
