@@ -131,22 +131,36 @@ static size_t variant_lds(const Variant& v, int E) {
 
 static std::string spec_source(const ftrh::Prep& pr, const Variant& v) {
   const ftrh::Graph& g = pr.g;
+  /* elimination order: internal vertices, then those with nonzero P rows */
+  std::vector<int> ord, pos(g.V);
+  std::vector<char> isext(g.V, 0);
+  for (int x : pr.ext) isext[x] = 1;
+  for (int u = 0; u < g.V; u++)
+    if (!isext[u]) ord.push_back(u);
+  for (int u = 0; u < g.V; u++)
+    if (isext[u]) ord.push_back(u);
+  for (int q = 0; q < g.V; q++) pos[ord[q]] = q;
+  auto arr = [](std::ostringstream& s, const char* name, const std::vector<int>& a) {
+    s << "  static constexpr int " << name << "[" << a.size() << "] = {";
+    for (size_t i = 0; i < a.size(); i++) s << (i ? "," : "") << a[i];
+    s << "};\n";
+  };
   std::ostringstream s;
-  s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size()
-    << ";\n  static constexpr int hi[" << g.E << "] = {";
-  for (int e = 0; e < g.E; e++) s << (e ? "," : "") << g.hi[e];
-  s << "};\n  static constexpr int lo[" << g.E << "] = {";
-  for (int e = 0; e < g.E; e++) s << (e ? "," : "") << g.lo[e];
-  s << "};\n  static constexpr int ext[" << std::max<size_t>(1, pr.ext.size()) << "] = {";
-  if (pr.ext.empty()) s << "0";
-  for (size_t a = 0; a < pr.ext.size(); a++) s << (a ? "," : "") << pr.ext[a];
-  s << "};\n};\nFTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
+  s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
+  arr(s, "hi", g.hi);
+  arr(s, "lo", g.lo);
+  arr(s, "ord", ord);
+  arr(s, "pos", pos);
+  s << "};\nFTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
   return s.str();
 }
 
 static const std::vector<std::string>& rtc_options() {
-  static const std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++17",
-                                             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  static const std::vector<std::string> o = [] {
+    const char* opt = getenv("FEYNTROP_AMD_RTC_OPT");   /* e.g. -O1, for experiments */
+    return std::vector<std::string>{"--offload-arch=gfx950", opt ? opt : "-O3", "-std=c++17",
+                                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  }();
   return o;
 }
 
@@ -202,7 +216,13 @@ static std::string code_object(const std::string& src, std::string* path_out) {
 }
 
 static std::string full_source(const ftrh::Prep& pr, const Variant& v) {
-  return std::string(ftr_embedded_device_source) + spec_source(pr, v);
+  std::string src = std::string(ftr_embedded_device_source) + spec_source(pr, v);
+  if (const char* d = getenv("FEYNTROP_AMD_DUMP_SRC")) {
+    char name[64];
+    snprintf(name, sizeof name, "/ftr_%016llx.hip", (unsigned long long)fnv1a(src));
+    std::ofstream(std::string(d) + name) << src;
+  }
+  return src;
 }
 
 static void fill_params(ftr_problem* p) {

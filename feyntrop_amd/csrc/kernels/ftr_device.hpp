@@ -8,9 +8,11 @@
  * verbatim in the runtime library for hipRTC specialisation of new graphs.
  *
  * Graph spec G (generated per graph):
- *   static constexpr int E, V, NEXT, LOOPS;     edges, vertices, |ext|, loops
+ *   static constexpr int E, V, NEXT;             edges, vertices, |ext|
  *   static constexpr int hi[E], lo[E];           edge endpoints, hi > lo
- *   static constexpr int ext[NEXT];              vertices with nonzero P rows
+ *   static constexpr int ord[V], pos[V];         elimination order: internal
+ *                                                vertices, then the NEXT
+ *                                                vertices with nonzero P rows
  *
  * Reference semantics followed (michibo/feyntrop):
  *   sampler   srcs/tropical_sampling.hpp:468-564
@@ -27,7 +29,8 @@
  *     det(X^-1 + i c H) * prod x, whose real part diag(1/x) is positive
  *     definite (Eigen: FullPivLU / PartialPivLU);
  *   - tr(La^-1 P) is taken over the external vertices only (P is zero on the
- *     others) from column solves.
+ *     others): internal vertices are eliminated first and the solves run on
+ *     the trailing external block.
  */
 #pragma once
 
@@ -75,13 +78,16 @@ __device__ inline void apply_poly(Xoshiro& g, const uint64_t* q) {
 /* ------------------------------------------------------------- complex -- */
 struct cplx {
   double re, im;
+  __host__ __device__ constexpr cplx() : re(0.0), im(0.0) {}
+  __host__ __device__ constexpr cplx(double r) : re(r), im(0.0) {}
+  __host__ __device__ constexpr cplx(double r, double i) : re(r), im(i) {}
 };
-__host__ __device__ inline cplx cmk(double a, double b) { return cplx{a, b}; }
+__host__ __device__ inline cplx cmk(double a, double b) { return cplx(a, b); }
 __host__ __device__ inline cplx operator+(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
 __host__ __device__ inline cplx operator-(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
 __host__ __device__ inline cplx operator-(cplx a) { return cplx{-a.re, -a.im}; }
 __host__ __device__ inline cplx operator*(cplx a, cplx b) {
-  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return cplx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 __host__ __device__ inline cplx operator*(double s, cplx a) { return cplx{s * a.re, s * a.im}; }
 __host__ __device__ inline cplx crecip(cplx a) {
@@ -160,6 +166,92 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
 }
 
 /* ----------------------------------------------------------- integrand -- */
+/* Matrices are stored lower-triangular-packed in elimination order: vertex
+ * G::ord[p] sits at position p (internal vertices first, the vertices that
+ * carry external momentum last; G::pos is the inverse).  Eliminating the
+ * internal vertices first leaves the Schur complement on the external block,
+ * so tr(La^-1 P) needs solves on that block only and the factors of the
+ * internal columns die right after their elimination step. */
+#define FTR_SYM(A, i, j) (((i) >= (j)) ? A[FTR_TRI(i, j)] : A[FTR_TRI(j, i)])
+
+/* In-place right-looking LDL^T of a packed symmetric matrix (real or
+ * complex-symmetric); returns the product of the pivots and their inverses
+ * in dinv. */
+template <int V>
+__host__ __device__ inline double ldlt_inplace(double (&M)[V * (V + 1) / 2], double (&dinv)[V]) {
+  double prod = 1.0;
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    const double d = M[FTR_TRI(k, k)];
+    prod *= d;
+    const double inv = 1.0 / d;
+    dinv[k] = inv;
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      const double aik = M[FTR_TRI(i, k)];
+#pragma unroll
+      for (int j = k + 1; j <= i; j++) M[FTR_TRI(i, j)] -= aik * (M[FTR_TRI(j, k)] * inv);
+    }
+#pragma unroll
+    for (int i = k + 1; i < V; i++) M[FTR_TRI(i, k)] *= inv;
+  }
+  return prod;
+}
+
+template <int V>
+__host__ __device__ inline cplx ldlt_inplace(cplx (&M)[V * (V + 1) / 2], cplx (&dinv)[V]) {
+  cplx prod = cmk(1.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    const cplx d = M[FTR_TRI(k, k)];
+    prod = prod * d;
+    const cplx inv = crecip(d);
+    dinv[k] = inv;
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      const cplx lik = M[FTR_TRI(i, k)] * inv;
+#pragma unroll
+      for (int j = k + 1; j <= i; j++) M[FTR_TRI(i, j)] = M[FTR_TRI(i, j)] - lik * M[FTR_TRI(j, k)];
+    }
+#pragma unroll
+    for (int i = k + 1; i < V; i++) M[FTR_TRI(i, k)] = M[FTR_TRI(i, k)] * inv;
+  }
+  return prod;
+}
+
+/* tr(A^-1 P) over the trailing external block from the factors of A (packed
+ * L below the diagonal, dinv); P entries at external vertices, the deleted
+ * vertex vs excluded.  Only positions >= V - NEXT are touched. */
+template <class G, class T>
+__host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], const T (&dinv)[G::V],
+                                         const DevParams& p, int vs) {
+  constexpr int V = G::V, N0 = G::V - G::NEXT;
+  T tr = T{};
+#pragma unroll
+  for (int a = 0; a < G::NEXT; a++) {
+    const int pj = N0 + a;               /* position of the column */
+    const int vj = G::ord[pj];
+    T y[V];
+#pragma unroll
+    for (int i = N0; i < V; i++) y[i] = (i == pj) ? T{1.0} : T{};
+#pragma unroll
+    for (int i = N0; i < V; i++)
+#pragma unroll
+      for (int q = pj; q < i; q++) y[i] = y[i] - M[FTR_TRI(i, q)] * y[q];
+#pragma unroll
+    for (int i = N0; i < V; i++) y[i] = y[i] * dinv[i];
+#pragma unroll
+    for (int i = V - 1; i >= N0; i--)
+#pragma unroll
+      for (int q = i + 1; q < V; q++) y[i] = y[i] - M[FTR_TRI(q, i)] * y[q];
+    T colsum = T{};
+#pragma unroll
+    for (int b = 0; b < G::NEXT; b++) colsum = colsum + p.P[G::ord[N0 + b] * FTR_MAX_V + vj] * y[N0 + b];
+    if (vj != vs) tr = tr + colsum;
+  }
+  return tr;
+}
+
 /* out[k] (complex, imag 0 on the real path).  Returns false when dropped
  * (non-finite R0, feynman_integral.hpp:227-234, :254-261, :329-347). */
 template <class G, int K, int MODE>
@@ -192,42 +284,28 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
     X[e] *= scale;
     Xi[e] = 1.0 / X[e];
   }
-  /* Laplacian with weights 1/x (laplacian.hpp:14-67), v* row/col -> identity */
+  double xprod = 1.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) xprod *= X[e];
+  /* Laplacian with weights 1/x (laplacian.hpp:14-67) in elimination order,
+   * v* row/col -> identity */
   double M[NT];
 #pragma unroll
   for (int q = 0; q < NT; q++) M[q] = 0.0;
 #pragma unroll
   for (int e = 0; e < E; e++) {
-    M[FTR_TRI(G::lo[e], G::lo[e])] += Xi[e];
-    M[FTR_TRI(G::hi[e], G::hi[e])] += Xi[e];
-    M[FTR_TRI(G::hi[e], G::lo[e])] -= Xi[e];
+    const int a = G::pos[G::lo[e]], b = G::pos[G::hi[e]];
+    M[FTR_TRI(a, a)] += Xi[e];
+    M[FTR_TRI(b, b)] += Xi[e];
+    M[(a > b) ? FTR_TRI(a, b) : FTR_TRI(b, a)] -= Xi[e];
   }
 #pragma unroll
   for (int i = 0; i < V; i++)
 #pragma unroll
     for (int j = 0; j <= i; j++)
-      if (i == vs || j == vs) M[FTR_TRI(i, j)] = (i == j) ? 1.0 : 0.0;
-  /* LDL^T, static order; M keeps L below the diagonal, dinv the inverse pivots */
-  double dprod = 1.0, dinv[V];
-#pragma unroll
-  for (int k = 0; k < V; k++) {
-    const double d = M[FTR_TRI(k, k)];
-    dprod *= d;
-    const double inv = 1.0 / d;
-    dinv[k] = inv;
-#pragma unroll
-    for (int i = k + 1; i < V; i++) {
-      const double aik = M[FTR_TRI(i, k)];
-#pragma unroll
-      for (int j = k + 1; j <= i; j++) M[FTR_TRI(i, j)] -= aik * (M[FTR_TRI(j, k)] * inv);
-    }
-#pragma unroll
-    for (int i = k + 1; i < V; i++) M[FTR_TRI(i, k)] *= inv;
-  }
-  double xprod = 1.0;
-#pragma unroll
-  for (int e = 0; e < E; e++) xprod *= X[e];
-  const double psi = dprod * xprod;
+      if (G::ord[i] == vs || G::ord[j] == vs) M[FTR_TRI(i, j)] = (i == j) ? 1.0 : 0.0;
+  double dinv[V];
+  const double psi = ldlt_inplace<V>(M, dinv) * xprod;
 
   if constexpr (MODE != 2) {
     double R = p.IGtr * pow(psi, p.mhalfD);
@@ -239,29 +317,7 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
       out[0] = cmk(R, 0.0);
       return true;
     }
-    /* tr(La^-1 P) over the external vertices from column solves */
-    double tr = 0.0;
-#pragma unroll
-    for (int a = 0; a < G::NEXT; a++) {
-      const int j = G::ext[a];
-      double y[V];
-#pragma unroll
-      for (int i = 0; i < V; i++) y[i] = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-      for (int i = 0; i < V; i++)
-#pragma unroll
-        for (int q = 0; q < i; q++) y[i] -= M[FTR_TRI(i, q)] * y[q];
-#pragma unroll
-      for (int i = 0; i < V; i++) y[i] *= dinv[i];
-#pragma unroll
-      for (int i = V - 1; i >= 0; i--)
-#pragma unroll
-        for (int q = i + 1; q < V; q++) y[i] -= M[FTR_TRI(q, i)] * y[q];
-      double colsum = 0.0;
-#pragma unroll
-      for (int b = 0; b < G::NEXT; b++) colsum += y[G::ext[b]] * p.P[G::ext[b] * FTR_MAX_V + j];
-      tr += (j == vs) ? 0.0 : colsum;
-    }
+    const double tr = trace_inv_P<G, double>(M, dinv, p, vs);
     double dot = 0.0;
 #pragma unroll
     for (int e = 0; e < E; e++) dot += p.m2[e] * X[e];
@@ -280,67 +336,64 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
     return true;
   } else {
     /* ---- analytic continuation (feynman_integral.hpp:278-362) ---- */
-    /* Gm = reduced inverse, zero on v*: Gm = L^-T D^-1 L^-1 */
-    double Li[NT]; /* L^-1, unit lower (diagonal unused) */
-#pragma unroll
-    for (int i = 0; i < V; i++)
-#pragma unroll
-      for (int j = 0; j < i; j++) {
-        double s = -M[FTR_TRI(i, j)];
-#pragma unroll
-        for (int q = j + 1; q < i; q++) s -= M[FTR_TRI(i, q)] * Li[FTR_TRI(q, j)];
-        Li[FTR_TRI(i, j)] = s;
-      }
+    /* Gm = reduced inverse, zero on v*: Gm = L^-T D^-1 L^-1 (positions) */
     double Gm[NT];
+    {
+      double Li[NT]; /* L^-1, unit lower (diagonal unused) */
 #pragma unroll
-    for (int i = 0; i < V; i++)
+      for (int i = 0; i < V; i++)
 #pragma unroll
-      for (int j = 0; j <= i; j++) {
-        double s = 0.0;
+        for (int j = 0; j < i; j++) {
+          double s = -M[FTR_TRI(i, j)];
 #pragma unroll
-        for (int k = i; k < V; k++) {
-          const double lki = (k == i) ? 1.0 : Li[FTR_TRI(k, i)];
-          const double lkj = (k == j) ? 1.0 : Li[FTR_TRI(k, j)];
-          s += lki * lkj * dinv[k];
+          for (int q = j + 1; q < i; q++) s -= M[FTR_TRI(i, q)] * Li[FTR_TRI(q, j)];
+          Li[FTR_TRI(i, j)] = s;
         }
-        Gm[FTR_TRI(i, j)] = (i == vs || j == vs) ? 0.0 : s;
-      }
-    /* LPGPLT = Gm P Gm over the external support */
-    double Y[V][G::NEXT];
 #pragma unroll
-    for (int u = 0; u < V; u++)
+      for (int i = 0; i < V; i++)
 #pragma unroll
-      for (int b = 0; b < G::NEXT; b++) {
-        double s = 0.0;
+        for (int j = 0; j <= i; j++) {
+          double s = 0.0;
 #pragma unroll
-        for (int a = 0; a < G::NEXT; a++) {
-          const int i = G::ext[a];
-          const double gui = (u >= i) ? Gm[FTR_TRI(u, i)] : Gm[FTR_TRI(i, u)];
-          s += gui * p.P[i * FTR_MAX_V + G::ext[b]];
+          for (int k = i; k < V; k++) {
+            const double lki = (k == i) ? 1.0 : Li[FTR_TRI(k, i)];
+            const double lkj = (k == j) ? 1.0 : Li[FTR_TRI(k, j)];
+            s += lki * lkj * dinv[k];
+          }
+          Gm[FTR_TRI(i, j)] = (G::ord[i] == vs || G::ord[j] == vs) ? 0.0 : s;
         }
-        Y[u][b] = s;
-      }
+    }
+    /* LPGPLT = Gm P Gm, only needed through P's external support */
     double LP[NT];
+    {
+      constexpr int N0 = V - G::NEXT;
+      double Y[V][G::NEXT > 0 ? G::NEXT : 1];
 #pragma unroll
-    for (int u = 0; u < V; u++)
-#pragma unroll
-      for (int w = 0; w <= u; w++) {
-        double s = 0.0;
+      for (int u = 0; u < V; u++)
 #pragma unroll
         for (int b = 0; b < G::NEXT; b++) {
-          const int j = G::ext[b];
-          const double gjw = (j >= w) ? Gm[FTR_TRI(j, w)] : Gm[FTR_TRI(w, j)];
-          s += Y[u][b] * gjw;
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < G::NEXT; a++)
+            s += FTR_SYM(Gm, u, N0 + a) * p.P[G::ord[N0 + a] * FTR_MAX_V + G::ord[N0 + b]];
+          Y[u][b] = s;
         }
-        LP[FTR_TRI(u, w)] = s;
-      }
-#define FTR_SYM(A, i, j) (((i) >= (j)) ? A[FTR_TRI(i, j)] : A[FTR_TRI(j, i)])
+#pragma unroll
+      for (int u = 0; u < V; u++)
+#pragma unroll
+        for (int w = 0; w <= u; w++) {
+          double s = 0.0;
+#pragma unroll
+          for (int b = 0; b < G::NEXT; b++) s += Y[u][b] * FTR_SYM(Gm, N0 + b, w);
+          LP[FTR_TRI(u, w)] = s;
+        }
+    }
     /* first and second derivatives of pphi (laplacian.hpp:102-215) */
     double th[E];
     double dd[E * (E + 1) / 2];
 #pragma unroll
     for (int a = 0; a < E; a++) {
-      const int ka = G::hi[a], la = G::lo[a];
+      const int ka = G::pos[G::hi[a]], la = G::pos[G::lo[a]];
       const double s = FTR_SYM(LP, ka, ka) + FTR_SYM(LP, la, la) - 2.0 * FTR_SYM(LP, ka, la);
       const double t = FTR_SYM(Gm, ka, ka) + FTR_SYM(Gm, la, la) - 2.0 * FTR_SYM(Gm, ka, la);
       const double xa = Xi[a] * Xi[a];
@@ -349,13 +402,42 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
       dd[FTR_TRI(a, a)] = 2.0 * xas * (Xi[a] - xa * t);
 #pragma unroll
       for (int b = 0; b < a; b++) {
-        const int kb = G::hi[b], lb = G::lo[b];
+        const int kb = G::pos[G::hi[b]], lb = G::pos[G::lo[b]];
         const double s2 = FTR_SYM(LP, ka, kb) + FTR_SYM(LP, la, lb) - FTR_SYM(LP, ka, lb) - FTR_SYM(LP, la, kb);
         const double t2 = FTR_SYM(Gm, ka, kb) + FTR_SYM(Gm, la, lb) - FTR_SYM(Gm, ka, lb) - FTR_SYM(Gm, la, kb);
         const double xb = Xi[b] * Xi[b];
         dd[FTR_TRI(a, b)] = -2.0 * xb * xa * s2 * t2;
       }
     }
+    /* deformation Jacobian: det(I + i c H X) = det(X^-1 + i c H) prod x.
+     * Left-looking (Crout) LDL^T of the complex-symmetric B = X^-1 + i c H:
+     * column j needs rows >= j of the finished columns only, so the factor
+     * rows die as the sweep advances; only the pivots are kept. */
+    cplx jdet = cmk(1.0, 0.0);
+    {
+      cplx L[E * (E + 1) / 2];  /* L[i][k] for k < i; row i dies after column i */
+      cplx d[E];
+#pragma unroll
+      for (int j = 0; j < E; j++) {
+        cplx w[E];              /* w_k = d_k L[j][k] */
+#pragma unroll
+        for (int k = 0; k < j; k++) w[k] = d[k] * L[FTR_TRI(j, k)];
+        cplx dj = cmk(Xi[j], p.def_ref * dd[FTR_TRI(j, j)]);
+#pragma unroll
+        for (int k = 0; k < j; k++) dj = dj - L[FTR_TRI(j, k)] * w[k];
+        d[j] = dj;
+        jdet = jdet * dj;
+        const cplx inv = crecip(dj);
+#pragma unroll
+        for (int i = j + 1; i < E; i++) {
+          cplx v = cmk(0.0, p.def_ref * dd[FTR_TRI(i, j)]);
+#pragma unroll
+          for (int k = 0; k < j; k++) v = v - L[FTR_TRI(i, k)] * w[k];
+          L[FTR_TRI(i, j)] = v * inv;
+        }
+      }
+    }
+    jdet = xprod * jdet;
     /* deformation: theta = def_ref (d pphi + m^2), z = x e^{i theta} */
     cplx cX[E], cXi[E];
     double xne = 0.0;
@@ -368,95 +450,46 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
       cXi[e] = cmk(Xi[e] * cs, -Xi[e] * sn);
       xne += p.nu[e] * th[e];
     }
-    /* complex Laplacian La(1/z), v* -> identity, complex-symmetric LDL^T */
-    cplx CM[NT];
+    /* complex Laplacian La(1/z) in elimination order, v* -> identity,
+     * complex-symmetric LDL^T and tr(La(1/z)^-1 P) on the external block */
+    cplx cdet, ctr;
+    {
+      cplx CM[NT], cdinv[V];
 #pragma unroll
-    for (int q = 0; q < NT; q++) CM[q] = cmk(0.0, 0.0);
+      for (int q = 0; q < NT; q++) CM[q] = cmk(0.0, 0.0);
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      CM[FTR_TRI(G::lo[e], G::lo[e])] = CM[FTR_TRI(G::lo[e], G::lo[e])] + cXi[e];
-      CM[FTR_TRI(G::hi[e], G::hi[e])] = CM[FTR_TRI(G::hi[e], G::hi[e])] + cXi[e];
-      CM[FTR_TRI(G::hi[e], G::lo[e])] = CM[FTR_TRI(G::hi[e], G::lo[e])] - cXi[e];
-    }
-#pragma unroll
-    for (int i = 0; i < V; i++)
-#pragma unroll
-      for (int j = 0; j <= i; j++)
-        if (i == vs || j == vs) CM[FTR_TRI(i, j)] = cmk((i == j) ? 1.0 : 0.0, 0.0);
-    cplx cdet = cmk(1.0, 0.0), cdinv[V];
-#pragma unroll
-    for (int k = 0; k < V; k++) {
-      const cplx d = CM[FTR_TRI(k, k)];
-      cdet = cdet * d;
-      const cplx inv = crecip(d);
-      cdinv[k] = inv;
-#pragma unroll
-      for (int i = k + 1; i < V; i++) {
-        const cplx lik = CM[FTR_TRI(i, k)] * inv;
-#pragma unroll
-        for (int j = k + 1; j <= i; j++) CM[FTR_TRI(i, j)] = CM[FTR_TRI(i, j)] - lik * CM[FTR_TRI(j, k)];
+      for (int e = 0; e < E; e++) {
+        const int a = G::pos[G::lo[e]], b = G::pos[G::hi[e]];
+        const int o = (a > b) ? FTR_TRI(a, b) : FTR_TRI(b, a);
+        CM[FTR_TRI(a, a)] = CM[FTR_TRI(a, a)] + cXi[e];
+        CM[FTR_TRI(b, b)] = CM[FTR_TRI(b, b)] + cXi[e];
+        CM[o] = CM[o] - cXi[e];
       }
-#pragma unroll
-      for (int i = k + 1; i < V; i++) CM[FTR_TRI(i, k)] = CM[FTR_TRI(i, k)] * inv;
-    }
-    cplx cxp = cmk(1.0, 0.0);
-#pragma unroll
-    for (int e = 0; e < E; e++) cxp = cxp * cX[e];
-    const cplx cPsi = cxp * cdet;
-    cplx cR = p.IGtr * cpow_real(cPsi, p.mhalfD);
-    /* deformation Jacobian: det(I + i c H X) = det(X^-1 + i c H) prod x */
-    cplx B[E * (E + 1) / 2];
-#pragma unroll
-    for (int a = 0; a < E; a++)
-#pragma unroll
-      for (int b = 0; b <= a; b++)
-        B[FTR_TRI(a, b)] = cmk((a == b) ? Xi[a] : 0.0, p.def_ref * dd[FTR_TRI(a, b)]);
-    cplx jdet = cmk(1.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < E; k++) {
-      const cplx d = B[FTR_TRI(k, k)];
-      jdet = jdet * d;
-      const cplx inv = crecip(d);
-#pragma unroll
-      for (int i = k + 1; i < E; i++) {
-        const cplx lik = B[FTR_TRI(i, k)] * inv;
-#pragma unroll
-        for (int j = k + 1; j <= i; j++) B[FTR_TRI(i, j)] = B[FTR_TRI(i, j)] - lik * B[FTR_TRI(j, k)];
-      }
-    }
-    jdet = xprod * jdet;
-    double sn, cs;
-    sincos(xne, &sn, &cs);
-    cR = cR * (jdet * cmk(cs, sn));
-    /* tr(La(1/z)^-1 P) over the external support */
-    cplx ctr = cmk(0.0, 0.0);
-#pragma unroll
-    for (int a = 0; a < G::NEXT; a++) {
-      const int j = G::ext[a];
-      cplx y[V];
-#pragma unroll
-      for (int i = 0; i < V; i++) y[i] = cmk((i == j) ? 1.0 : 0.0, 0.0);
 #pragma unroll
       for (int i = 0; i < V; i++)
 #pragma unroll
-        for (int q = 0; q < i; q++) y[i] = y[i] - CM[FTR_TRI(i, q)] * y[q];
-#pragma unroll
-      for (int i = 0; i < V; i++) y[i] = y[i] * cdinv[i];
-#pragma unroll
-      for (int i = V - 1; i >= 0; i--)
-#pragma unroll
-        for (int q = i + 1; q < V; q++) y[i] = y[i] - CM[FTR_TRI(q, i)] * y[q];
-      cplx colsum = cmk(0.0, 0.0);
-#pragma unroll
-      for (int b = 0; b < G::NEXT; b++) colsum = colsum + p.P[G::ext[b] * FTR_MAX_V + j] * y[G::ext[b]];
-      if (j != vs) ctr = ctr + colsum;
+        for (int j = 0; j <= i; j++)
+          if (G::ord[i] == vs || G::ord[j] == vs) CM[FTR_TRI(i, j)] = cmk((i == j) ? 1.0 : 0.0, 0.0);
+      cdet = ldlt_inplace<V>(CM, cdinv);
+      ctr = trace_inv_P<G, cplx>(CM, cdinv, p, vs);
     }
-    cplx cdot = cmk(0.0, 0.0);
+    cplx cxp = cmk(1.0, 0.0), cdot = cmk(0.0, 0.0);
 #pragma unroll
-    for (int e = 0; e < E; e++) cdot = cdot + p.m2[e] * cX[e];
+    for (int e = 0; e < E; e++) {
+      cxp = cxp * cX[e];
+      cdot = cdot + p.m2[e] * cX[e];
+    }
+    const cplx cPsi = cxp * cdet;
     const cplx cPXi = cdot - ctr;
+    cplx cR = p.IGtr * cpow_real(cPsi, p.mhalfD);
+    double sn, cs;
+    sincos(xne, &sn, &cs);
+    cR = cR * (jdet * cmk(cs, sn));
     cR = cR * cpow_real(cPXi, p.mW);
-    if (!cfinite(cR)) return false;
+    /* Branch-free tail: the finiteness flag is returned, not branched on
+     * (an early return here made the gfx950 backend at -O2/-O3 produce wrong
+     * values for the 10-edge ladder under high register pressure). */
+    const bool ok = cfinite(cR);
     out[0] = cR;
     if (K > 1) {
       const cplx lp = clog_(cPsi), lx = clog_(cPXi);
@@ -467,10 +500,10 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
         out[k] = cR;
       }
     }
-    return true;
-#undef FTR_SYM
+    return ok;
   }
 }
+#undef FTR_SYM
 
 /* -------------------------------------------------------- table views -- */
 /* TMODE 0: whole table in LDS; 1: w = J/r in LDS, the rest from global
