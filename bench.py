@@ -73,6 +73,58 @@ def cpu_baseline(prob, n_samples, threads):
                       f"('seconds sampling' {secs:.2f} s, wall {wall:.2f} s)"}
 
 
+def run_scan(eng, n_points, seed=0):
+    """Batched scan: n_points kinematic points (1- to 3-loop topologies,
+    Euclidean and Minkowski, N log-uniform in [1e5, 1e7]) in one hipGraph of
+    multi-problem kernels.  Returns throughput over the whole batch."""
+    import feyntrop_amd
+    from feyntrop_amd import workloads as W
+    probs = []
+
+    def accept(prob):
+        try:
+            probs.append(eng.prepare(prob))
+            return True
+        except feyntrop_amd.FeyntropError:
+            return False
+    t0 = time.perf_counter()
+    pts = W.scan_points(n_points, seed, accept=accept)
+    t_prep = time.perf_counter() - t0
+    Ns = [p["N"] for p in pts]
+    seeds = [p["seed"] for p in pts]
+    eng.integrate_batch(probs[:8], [10000] * 8, seeds[:8])       # warm-up
+    t0 = time.perf_counter()
+    outs = eng.integrate_batch(probs, Ns, seeds)
+    wall = time.perf_counter() - t0
+    tot = float(sum(Ns))
+    paths = {}
+    for q in probs:
+        paths[q.info.path] = paths.get(q.info.path, 0) + 1
+    return {"workload": f"batched scan: {len(pts)} points, {len(set(p['_topology'] for p in pts))} topologies",
+            "samples": tot, "value": tot / wall, "unit": "samples/s", "graph_ms": outs[0].kernel_ms,
+            "wall_s": wall, "prep_s": t_prep, "paths": paths,
+            "dropped": int(sum(m.dropped for m in outs))}
+
+
+def run_tuned(eng, name, steps=2):
+    """A named workload with the autotuner's kernel variant."""
+    from feyntrop_amd import workloads as W
+    prob = W.all_problems()[name]()
+    p = eng.prepare(prob)
+    tm, blk, _ = eng.autotune(p, n_probe=min(int(prob["N"]), 20_000_000))
+    eng.integrate(p, prob["N"], 1)
+    t0 = time.perf_counter()
+    ms = []
+    for i in range(steps):
+        ms.append(eng.integrate(p, prob["N"], 100 + i).kernel_ms)
+    wall = time.perf_counter() - t0
+    fps = flops_per_sample(p.info)
+    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}", "N_per_step": prob["N"],
+            "value": prob["N"] * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
+            "variant": {"table_mode": tm, "block": blk},
+            "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +135,8 @@ def main():
     ap.add_argument("--cpu-samples", type=float, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--scan-points", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -136,6 +190,16 @@ def main():
                      "kernel": "ftrk_int", "flops_per_sample": fps, "kernel_ms_mean": mean_kms},
         "learned": None,
     }
+    if not args.no_extras:
+        extras = {}
+        for key, fn in (("scan", lambda: run_scan(eng, args.scan_points)),
+                        ("cubic5", lambda: run_tuned(eng, "cubic5")),
+                        ("triple_ladder", lambda: run_tuned(eng, "triple_ladder"))):
+            try:
+                extras[key] = fn()
+            except Exception as e:  # an extra must not take the main line down
+                extras[key] = {"error": str(e)[:300]}
+        line["extra_workloads"] = extras
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ns = args.cpu_samples or 2.5e6 * threads

@@ -92,6 +92,9 @@ def lib():
     L.ftr_integrate_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_uint64), C.POINTER(ftr_moments)]
     L.ftr_set_cache_dir.argtypes = [C.c_char_p]
+    L.ftr_autotune.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.ftr_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
+    L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     _lib = L
     return L
@@ -128,6 +131,29 @@ def precompile(prob):
     rc = lib().ftr_precompile(C.byref(inp.s), err, 8192)
     if rc:
         raise FeyntropError(rc, err.value.decode(errors="replace"))
+
+
+def host_prepare(prob):
+    """Host-only preparation (no GPU): validation, regime flags, J-table."""
+    inp = _Input(prob)
+    out = C.c_void_p()
+    rc = lib().ftr_prepare(None, C.byref(inp.s), C.byref(out))
+    if rc:
+        raise FeyntropError(rc, lib().ftr_last_error(None).decode(errors="replace"))
+    return Problem(None, out, prob)
+
+
+def host_path(prob):
+    """Kernel path of a problem (0 real, 1 real psi-only, 2 deformed) or None
+    if the preprocessing rejects it."""
+    try:
+        return host_prepare(prob).info.path
+    except FeyntropError:
+        return None
+
+
+def host_accepts(prob):
+    return host_path(prob) is not None
 
 
 class Problem:
@@ -169,8 +195,11 @@ class Engine:
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().ftr_destroy(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.ftr_destroy(self.h)
+            except Exception:
+                pass
             self.h = None
 
     def __del__(self):
@@ -217,6 +246,23 @@ class Engine:
                               C.c_void_p(out_ptr), C.c_void_p(drop_ptr))
         if rc:
             raise self._err(rc)
+
+    def autotune(self, p, n_probe=2_000_000):
+        tm, blk, ms = C.c_int(), C.c_int(), C.c_double()
+        rc = lib().ftr_autotune(self.h, p.h, int(n_probe), C.byref(tm), C.byref(blk), C.byref(ms))
+        if rc:
+            raise self._err(rc)
+        return tm.value, blk.value, ms.value
+
+    def set_variant(self, p, table_mode, block):
+        rc = lib().ftr_set_variant(self.h, p.h, int(table_mode), int(block))
+        if rc:
+            raise self._err(rc)
+
+    def variant(self, p):
+        tm, blk = C.c_int(), C.c_int()
+        lib().ftr_get_variant(p.h, C.byref(tm), C.byref(blk))
+        return tm.value, blk.value
 
     def run_json(self, text):
         out, log = C.c_void_p(), C.c_void_p()

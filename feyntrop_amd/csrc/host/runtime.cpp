@@ -42,7 +42,7 @@ struct Variant {
 
 struct Module {
   hipModule_t mod = nullptr;
-  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr;
+  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr, f_bat = nullptr;
 };
 
 struct ftr_ctx {
@@ -56,11 +56,17 @@ struct ftr_ctx {
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   std::vector<double> h_partials;
+  /* batched scans */
+  void* d_batch = nullptr;
+  size_t batch_bytes = 0;
+  double* h_pinned = nullptr; /* pinned copy-back buffer of the batch graph */
+  size_t pinned_bytes = 0;
 };
 
 struct ftr_problem {
   ftrh::Prep prep;
   ftr_ctx* ctx = nullptr;
+  int device = 0;
   Variant var;
   size_t lds = 0;
   Module m;
@@ -116,11 +122,38 @@ static Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
   const int E = pr.g.E;
   const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
   const size_t cap0 = std::min<size_t>(lds_limit, 64 * 1024);
-  if (full <= cap0) v.tmode = 0;
-  else if (wonly <= lds_limit) v.tmode = 1;
-  else v.tmode = 2;
   v.block = 256;
+  if (full <= cap0) {
+    v.tmode = 0;
+  } else if (wonly <= lds_limit) {
+    /* one table copy fills most of the LDS, so one block per CU: make it a
+     * big one on the real path (the deformed path needs > 128 VGPRs) */
+    v.tmode = 1;
+    if (pr.path != 2 && wonly + 16 * (2 + 4 * pr.K) * 8 <= lds_limit) v.block = 1024;
+  } else {
+    v.tmode = 2;
+  }
   return v;
+}
+
+/* Feasible (table placement, block size) pairs for the autotuner. */
+static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_limit) {
+  std::vector<Variant> out;
+  const int E = pr.g.E;
+  const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
+  for (int tm = 0; tm < 3; tm++)
+    for (int blk : {256, 512, 1024}) {
+      const size_t t = tm == 0 ? full : (tm == 1 ? wonly : 0);
+      if (t + (size_t)(blk / 64) * (2 + 4 * pr.K) * 8 + 16 > lds_limit) continue;
+      if (tm == 0 && full > 64 * 1024 && blk == 256) continue;
+      Variant v;
+      v.K = pr.K;
+      v.mode = pr.path;
+      v.tmode = tm;
+      v.block = blk;
+      out.push_back(v);
+    }
+  return out;
 }
 
 static size_t variant_lds(const Variant& v, int E) {
@@ -158,8 +191,14 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v) {
 static const std::vector<std::string>& rtc_options() {
   static const std::vector<std::string> o = [] {
     const char* opt = getenv("FEYNTROP_AMD_RTC_OPT");   /* e.g. -O1, for experiments */
-    return std::vector<std::string>{"--offload-arch=gfx950", opt ? opt : "-O3", "-std=c++17",
-                                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    std::vector<std::string> v{"--offload-arch=gfx950", opt ? opt : "-O3", "-std=c++17",
+                               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    if (const char* x = getenv("FEYNTROP_AMD_RTC_EXTRA")) {  /* extra options, space separated */
+      std::istringstream is(x);
+      std::string t;
+      while (is >> t) v.push_back(t);
+    }
+    return v;
   }();
   return o;
 }
@@ -215,6 +254,31 @@ static std::string code_object(const std::string& src, std::string* path_out) {
   return co;
 }
 
+/* Tuning cache: the autotuner's choice per (graph, K, path), next to the
+ * code objects. */
+static std::string tune_path(const ftrh::Prep& pr) {
+  Variant v;
+  v.K = pr.K;
+  v.mode = pr.path;
+  v.tmode = 9;
+  v.block = 0;
+  char name[64];
+  snprintf(name, sizeof name, "/tune_%016llx.txt", (unsigned long long)fnv1a(spec_source(pr, v)));
+  return cache_dir() + name;
+}
+
+static bool load_tuned(const ftrh::Prep& pr, size_t lds_limit, Variant* out) {
+  std::ifstream f(tune_path(pr));
+  int tm = -1, blk = 0;
+  if (!(f >> tm >> blk)) return false;
+  for (const Variant& v : candidate_variants(pr, lds_limit))
+    if (v.tmode == tm && v.block == blk) {
+      *out = v;
+      return true;
+    }
+  return false;
+}
+
 static std::string full_source(const ftrh::Prep& pr, const Variant& v) {
   std::string src = std::string(ftr_embedded_device_source) + spec_source(pr, v);
   if (const char* d = getenv("FEYNTROP_AMD_DUMP_SRC")) {
@@ -237,6 +301,9 @@ static void fill_params(ftr_problem* p) {
   d.IGtr = pr.IGtr;
   d.def_ref = pr.def_ref;
   d.inv_dlw = 1. / (pr.D / 2. * pr.L + pr.W);
+  const double mD = -pr.D / 2., mW = -pr.W;
+  if (mD == std::nearbyint(mD) && std::fabs(mD) <= 64) { d.iD = mD; d.use_iD = 1; }
+  if (mW == std::nearbyint(mW) && std::fabs(mW) <= 64) { d.iW = mW; d.use_iW = 1; }
   for (int e = 0; e < pr.g.E; e++) {
     d.nu[e] = pr.g.nu[e];
     d.m2[e] = pr.m2[e];
@@ -279,6 +346,7 @@ static Module load_module(ftr_ctx* c, const std::string& src) {
   HIPCHK(hipModuleGetFunction(&m.f_int, m.mod, "ftrk_int"));
   HIPCHK(hipModuleGetFunction(&m.f_rep, m.mod, "ftrk_rep"));
   HIPCHK(hipModuleGetFunction(&m.f_smp, m.mod, "ftrk_smp"));
+  HIPCHK(hipModuleGetFunction(&m.f_bat, m.mod, "ftrk_bat"));
   c->modules[src] = m;
   return m;
 }
@@ -382,6 +450,8 @@ void ftr_destroy(ftr_ctx* c) {
   (void)hipSetDevice(c->device);
   for (auto& kv : c->modules) (void)hipModuleUnload(kv.second.mod);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->d_batch) (void)hipFree(c->d_batch);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -421,6 +491,7 @@ int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
   *out = nullptr;
   ftr_problem* p = new ftr_problem();
   p->ctx = c;
+  p->device = c ? c->device : 0;
   try {
     const auto t0 = std::chrono::steady_clock::now();
     prep_from_input(p->prep, in);
@@ -431,7 +502,8 @@ int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
       return FTR_OK;
     }
     HIPCHK(hipSetDevice(c->device));
-    p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock);
+    if (!load_tuned(p->prep, c->prop.sharedMemPerBlock, &p->var))
+      p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock);
     p->lds = variant_lds(p->var, p->prep.g.E);
     p->m = load_module(c, full_source(p->prep, p->var));
     if (p->lds > 64 * 1024)
@@ -470,7 +542,8 @@ int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
 
 void ftr_problem_free(ftr_problem* p) {
   if (!p) return;
-  if (p->ctx) (void)hipSetDevice(p->ctx->device);
+  /* the context may already be gone: only the device id is used */
+  if (p->d_prm) (void)hipSetDevice(p->device);
   if (p->d_prm) (void)hipFree(p->d_prm);
   if (p->d_w) (void)hipFree(p->d_w);
   if (p->d_jr) (void)hipFree(p->d_jr);
@@ -635,6 +708,264 @@ int ftr_replay(ftr_ctx* c, ftr_problem* p, uint64_t n, const double* X_dev, cons
     const Error* fe = dynamic_cast<const Error*>(&e);
     return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
   }
+  return FTR_OK;
+}
+
+/* Batched scan: problems sharing a kernel (same graph spec, K, path and
+ * variant) form a group served by one launch of the multi-problem kernel;
+ * every block works on one problem.  All groups and the copy-back of the
+ * partial moments are captured in one hipGraph and launched once. */
+int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint64_t* Ns, const uint64_t* seeds,
+                        ftr_moments* out) {
+  if (!c || n < 0 || (n > 0 && (!probs || !Ns || !seeds || !out))) return FTR_ERR_INVALID;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(c->device));
+    for (int i = 0; i < n; i++) {
+      if (!probs[i] || probs[i]->ctx != c || !probs[i]->m.f_bat)
+        throw Error(FTR_ERR_INVALID, "batch: every problem must be prepared on this context");
+      memset(&out[i], 0, sizeof out[i]);
+      out[i].K = probs[i]->prep.K;
+    }
+    /* groups by kernel module */
+    std::map<hipFunction_t, std::vector<int>> groups;
+    for (int i = 0; i < n; i++)
+      if (Ns[i] > 0) groups[probs[i]->m.f_bat].push_back(i);
+    struct G {
+      hipFunction_t f;
+      int block;
+      size_t lds;
+      uint64_t S;
+      std::vector<int> idx;
+      size_t item_off, poly_off;
+      uint32_t nblocks;
+    };
+    std::vector<G> gs;
+    size_t n_items = 0, n_states = 0, n_polys = 0, n_part = 0;
+    std::vector<size_t> state_off(n, 0), part_off(n, 0);
+    std::vector<uint32_t> nb(n, 0);
+    for (auto& kv : groups) {
+      G g;
+      g.f = kv.first;
+      g.idx = kv.second;
+      ftr_problem* p0 = probs[g.idx[0]];
+      g.block = p0->var.block;
+      g.lds = p0->lds;
+      int nbpc = 0;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, g.f, g.block, g.lds) != hipSuccess || nbpc < 1)
+        nbpc = 1;
+      const uint64_t lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * g.block;
+      uint64_t tot = 0;
+      for (int i : g.idx) tot += Ns[i];
+      /* enough samples per lane to amortise the stream positioning, enough
+       * lanes for every problem */
+      g.S = std::max<uint64_t>(16, (tot + lanes_target - 1) / lanes_target);
+      g.item_off = n_items;
+      n_items += g.idx.size();
+      int lb = 0;
+      while ((1 << lb) < g.block) lb++;
+      g.poly_off = n_polys;
+      n_polys += (size_t)std::max(lb, 1);
+      g.nblocks = 0;
+      for (int i : g.idx) {
+        const uint64_t lanes = (Ns[i] + g.S - 1) / g.S;
+        nb[i] = (uint32_t)((lanes + g.block - 1) / g.block);
+        state_off[i] = n_states;
+        n_states += nb[i];
+        part_off[i] = n_part;
+        n_part += (size_t)nb[i] * (2 + 4 * probs[i]->prep.K);
+        g.nblocks += nb[i];
+      }
+      gs.push_back(g);
+    }
+    /* device layout: items | states | polys | partials */
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_items = al(n_items * sizeof(ftr::BatchItem)), b_states = al(n_states * 32),
+                 b_polys = al(n_polys * 32 + 32), b_part = al(n_part * 8 + 8);
+    const size_t need = b_items + b_states + b_polys + b_part;
+    if (need > c->batch_bytes) {
+      if (c->d_batch) HIPCHK(hipFree(c->d_batch));
+      c->d_batch = nullptr;
+      HIPCHK(hipMalloc(&c->d_batch, need));
+      c->batch_bytes = need;
+    }
+    char* base = (char*)c->d_batch;
+    ftr::BatchItem* d_items = (ftr::BatchItem*)base;
+    uint64_t* d_states = (uint64_t*)(base + b_items);
+    uint64_t* d_polys = (uint64_t*)(base + b_items + b_states);
+    double* d_part = (double*)(base + b_items + b_states + b_polys);
+    std::vector<ftr::BatchItem> items(n_items);
+    std::vector<uint64_t> states(4 * n_states), polys(4 * n_polys, 0);
+    for (auto& g : gs) {
+      const int E = probs[g.idx[0]]->prep.g.E;
+      const uint64_t per = 2 * (uint64_t)E - 1;
+      const unsigned __int128 dl = (unsigned __int128)g.S * per;
+      ftrh::Poly q = ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64));
+      int lb = 0;
+      while ((1 << lb) < g.block) lb++;
+      for (int j = 0; j < lb; j++) {
+        memcpy(&polys[4 * (g.poly_off + j)], q.w, 32);
+        q = ftrh::xo_poly_square(q);
+      }
+      const ftrh::Poly qb = q; /* one block: S * per * block */
+      uint32_t b0 = 0;
+      for (size_t t = 0; t < g.idx.size(); t++) {
+        const int i = g.idx[t];
+        ftr_problem* p = probs[i];
+        ftr::BatchItem& it = items[g.item_off + t];
+        memset(&it, 0, sizeof it);
+        it.a.prm = p->d_prm;
+        it.a.t_w = p->d_w;
+        it.a.t_jr = p->d_jr;
+        it.a.t_ir = p->d_ir;
+        it.a.t_mmL = p->d_mml;
+        it.a.block_state = d_states + 4 * state_off[i];
+        it.a.lane_poly = d_polys + 4 * g.poly_off;
+        it.a.N = Ns[i];
+        it.a.S = g.S;
+        it.a.partials = d_part + part_off[i];
+        it.block0 = b0;
+        it.nblocks = nb[i];
+        b0 += nb[i];
+      }
+      /* stream start state of every block of every problem */
+#pragma omp parallel for schedule(dynamic, 1)
+      for (long t = 0; t < (long)g.idx.size(); t++) {
+        const int i = g.idx[t];
+        uint64_t st[4];
+        ftrh::xo_seed(seeds[i], st);
+        for (uint32_t b = 0; b < nb[i]; b++) {
+          memcpy(&states[4 * (state_off[i] + b)], st, 32);
+          if (b + 1 < nb[i]) ftrh::xo_apply(qb, st);
+        }
+      }
+    }
+    if (n_items) {
+      HIPCHK(hipMemcpyHtoDAsync(d_items, items.data(), items.size() * sizeof(ftr::BatchItem), c->stream));
+      HIPCHK(hipMemcpyHtoDAsync(d_states, states.data(), states.size() * 8, c->stream));
+      HIPCHK(hipMemcpyHtoDAsync(d_polys, polys.data(), polys.size() * 8, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (n_part * 8 > c->pinned_bytes) {
+        if (c->h_pinned) HIPCHK(hipHostFree(c->h_pinned));
+        c->h_pinned = nullptr;
+        HIPCHK(hipHostMalloc((void**)&c->h_pinned, n_part * 8, hipHostMallocDefault));
+        c->pinned_bytes = n_part * 8;
+      }
+      double* h_part = c->h_pinned;
+      /* one graph: all group kernels, then the copy-back */
+      hipGraph_t graph;
+      hipGraphExec_t exec;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (auto& g : gs) {
+        const ftr::BatchItem* it = d_items + g.item_off;
+        int ni = (int)g.idx.size();
+        void* args[] = {(void*)&it, &ni};
+        HIPCHK(hipModuleLaunchKernel(g.f, g.nblocks, 1, 1, g.block, 1, 1, (unsigned)g.lds, c->stream, args, nullptr));
+      }
+      HIPCHK(hipMemcpyDtoHAsync(h_part, d_part, n_part * 8, c->stream));
+      HIPCHK(hipStreamEndCapture(c->stream, &graph));
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      HIPCHK(hipGraphLaunch(exec, c->stream));
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      for (auto& g : gs)
+        for (int i : g.idx) {
+          const int K = probs[i]->prep.K, NQ = 2 + 4 * K;
+          std::vector<double> tot(NQ, 0.0);
+          for (uint32_t b = 0; b < nb[i]; b++)
+            for (int q = 0; q < NQ; q++) tot[q] += h_part[part_off[i] + (size_t)b * NQ + q];
+          ftr_moments& m = out[i];
+          m.dropped = (uint64_t)tot[1];
+          for (int k = 0; k < K; k++) {
+            m.n[k][0] = m.n[k][1] = tot[0];
+            m.s1[k][0] = tot[2 + 4 * k];
+            m.s2[k][0] = tot[3 + 4 * k];
+            m.s1[k][1] = tot[4 + 4 * k];
+            m.s2[k][1] = tot[5 + 4 * k];
+          }
+          m.kernel_ms = ms; /* the whole graph */
+          m.lanes = (uint64_t)nb[i] * g.block;
+          m.samples_per_lane = g.S;
+        }
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < n; i++) {
+      if (probs[i]->prep.path != 2)
+        for (int k = 0; k < probs[i]->prep.K; k++) {
+          out[i].n[k][1] = 2;
+          out[i].s1[k][1] = 0;
+          out[i].s2[k][1] = 0;
+        }
+      out[i].seconds = secs;
+    }
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return FTR_OK;
+}
+
+int ftr_set_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block) {
+  if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock))
+      if (v.tmode == table_mode && v.block == block) {
+        p->m = load_module(c, full_source(p->prep, v));
+        p->var = v;
+        p->lds = variant_lds(v, p->prep.g.E);
+        if (p->lds > 64 * 1024)
+          (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
+        return FTR_OK;
+      }
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return fail(c, FTR_ERR_LIMIT, "infeasible (table_mode, block) for this problem");
+}
+
+int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block, double* best_ms) {
+  if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
+  const Variant keep = p->var;
+  double best = 1e300;
+  Variant bestv = keep;
+  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock)) {
+    if (ftr_set_variant(c, p, v.tmode, v.block) != FTR_OK) continue;
+    ftr_moments m;
+    double t = 0;
+    for (int rep = 0; rep < 2; rep++) {
+      if (ftr_integrate(c, p, n_probe, 12345, nullptr, &m) != FTR_OK) { t = 1e300; break; }
+      t = m.kernel_ms;
+    }
+    if (t < best) { best = t; bestv = v; }
+  }
+  if (best >= 1e300) {
+    ftr_set_variant(c, p, keep.tmode, keep.block);
+    return fail(c, FTR_ERR_DEVICE, "autotune: no variant ran");
+  }
+  const int rc = ftr_set_variant(c, p, bestv.tmode, bestv.block);
+  if (rc != FTR_OK) return rc;
+  {
+    mkdir(cache_dir().c_str(), 0755);
+    std::ofstream f(tune_path(p->prep));
+    if (f) f << bestv.tmode << " " << bestv.block << "\n";
+  }
+  if (table_mode) *table_mode = bestv.tmode;
+  if (block) *block = bestv.block;
+  if (best_ms) *best_ms = best;
+  return FTR_OK;
+}
+
+int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block) {
+  if (!p) return FTR_ERR_INVALID;
+  if (table_mode) *table_mode = p->var.tmode;
+  if (block) *block = p->var.block;
   return FTR_OK;
 }
 

@@ -29,6 +29,8 @@ struct DevParams {
   double IGtr;       /* J(G) */
   double def_ref;    /* -lambda_eff (0 on the real path) */
   double inv_dlw;    /* 1 / (D/2 L + W) */
+  double iD, iW;     /* -D/2 and -W when they are integers (|n| <= 64), else 0 */
+  double use_iD, use_iW; /* 1 when the integer powers apply */
   double nu[FTR_MAX_E];
   double m2[FTR_MAX_E];
   double P[FTR_MAX_V * FTR_MAX_V];
@@ -46,6 +48,14 @@ struct Launch {
   uint64_t N;              /* total samples */
   uint64_t S;              /* samples per lane */
   double* partials;        /* nblocks x (2 + 4K) */
+};
+
+/* One problem of a batched launch: its launch data plus the range of global
+ * blocks that work on it (items are sorted by block0). */
+struct BatchItem {
+  Launch a;
+  uint32_t block0;
+  uint32_t nblocks;
 };
 
 }  // namespace ftr

@@ -40,6 +40,10 @@
 #endif
 #include "ftr_abi.hpp"
 
+#ifndef FTR_SCAN_PARALLEL
+#define FTR_SCAN_PARALLEL 0
+#endif
+
 namespace ftr {
 
 /* ----------------------------------------------------------------- RNG -- */
@@ -110,26 +114,88 @@ __host__ __device__ inline cplx cpow_real(cplx z, double y) {
 }
 __host__ __device__ inline bool cfinite(cplx z) { return __builtin_isfinite(z.re) && __builtin_isfinite(z.im); }
 
+/* z^n for an integer n by repeated squaring (n uniform across the wave);
+ * the principal branch of pow(z, n) coincides with it for integer n. */
+__host__ __device__ inline cplx cipow(cplx z, int n) {
+  cplx r = cmk(1.0, 0.0), b = z;
+  unsigned m = n < 0 ? (unsigned)(-n) : (unsigned)n;
+  while (m) {
+    if (m & 1u) r = r * b;
+    m >>= 1;
+    if (m) b = b * b;
+  }
+  return n < 0 ? crecip(r) : r;
+}
+__host__ __device__ inline double dipow(double x, int n) {
+  double r = 1.0, b = x;
+  unsigned m = n < 0 ? (unsigned)(-n) : (unsigned)n;
+  while (m) {
+    if (m & 1u) r *= b;
+    m >>= 1;
+    if (m) b *= b;
+  }
+  return n < 0 ? 1.0 / r : r;
+}
+/* x^(-D/2), x^(-W) with the integer fast path when it applies */
+__host__ __device__ inline double pow_mD(const DevParams& p, double x) {
+  return p.use_iD != 0.0 ? dipow(x, (int)p.iD) : pow(x, p.mhalfD);
+}
+__host__ __device__ inline double pow_mW(const DevParams& p, double x) {
+  return p.use_iW != 0.0 ? dipow(x, (int)p.iW) : pow(x, p.mW);
+}
+__host__ __device__ inline cplx cpow_mD(const DevParams& p, cplx z) {
+  return p.use_iD != 0.0 ? cipow(z, (int)p.iD) : cpow_real(z, p.mhalfD);
+}
+__host__ __device__ inline cplx cpow_mW(const DevParams& p, cplx z) {
+  return p.use_iW != 0.0 ? cipow(z, (int)p.iW) : cpow_real(z, p.mW);
+}
+
 /* lower-triangular packed index for i >= j */
 #define FTR_TRI(i, j) ((i) * ((i) + 1) / 2 + (j))
 
 /* ------------------------------------------------------------- sampler -- */
 /* tropical_sampling.hpp:503-564.  Table access goes through the TableView
- * (LDS or global).  Returns psi_tr, xi_tr; X[e] filled in edge order. */
+ * (LDS or global).  X[e] filled in edge order; returns log psi_tr and log G. */
 template <class G, class TV>
-__device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& psi_tr,
-                                    double& xi_tr) {
+__device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& lpsi_tr,
+                                    double& lG) {
+  /* The radius product x = prod (1-U)^(1/r) is carried as its logarithm, so
+   * each radial draw costs one log and one exp (the reference: one pow); the
+   * samples agree with the reference's to ~1e-15 relative.  Returns
+   * log psi_tr = log F and log G, with xi_tr = G F. */
   constexpr int E = G::E;
   uint32_t g = (E == 32) ? 0xffffffffu : ((1u << E) - 1u);
   double JJr = tv.jr(g);
   uint32_t b0 = tv.mmL(g);
   bool MM = b0 & 1;
   int oL = (int)(b0 >> 1);
-  double Gv = 0.0, F = 1.0, XX = 1.0;
+  double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0;
 #pragma unroll
   for (int e = 0; e < E; e++) X[e] = 0.0;
   for (int j = 0; j < E; j++) {
     const double R = rng.uniform() * JJr;
+#if FTR_SCAN_PARALLEL
+    /* all candidate weights are loaded first (independent LDS reads), then
+     * the ascending running sum picks the first edge with T > R, or the last
+     * edge of the subgraph -- the same rule and summation order as the
+     * reference's visit_cuts */
+    double wv[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) wv[q] = ((g >> q) & 1u) ? tv.w(g ^ (1u << q)) : 0.0;
+    double T = 0.0;
+    int e = 31 - __builtin_clz(g);
+    bool found = false;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      if ((g >> q) & 1u) {
+        T += wv[q];
+        if (!found && T > R) {
+          found = true;
+          e = q;
+        }
+      }
+    }
+#else
     double T = 0.0;
     int e = 0;
     uint32_t m = g;
@@ -140,6 +206,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
       e = q;
       if (T > R) break;
     }
+#endif
     const uint32_t c = g ^ (1u << e);
     JJr = tv.jr(c);
     const double ir = tv.ir(c);
@@ -147,22 +214,23 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
     const bool mm = b & 1;
     const int L = (int)(b >> 1);
     g = c;
-    if (MM && !mm) Gv = XX;
+    if (MM && !mm) lGv = lX;
     MM = mm;
 #pragma unroll
     for (int k = 0; k < E; k++)
       if (k == e) X[k] = XX;
     if (L < oL) {
-      F *= XX;
+      lF += lX;
       oL = L;
     }
     if (j < E - 1) {
       const double U = rng.uniform();
-      XX *= pow(1. - U, ir);
+      lX += ir * log(1. - U);
+      XX = exp(lX);
     }
   }
-  psi_tr = F;
-  xi_tr = Gv * F;
+  lpsi_tr = lF;
+  lG = lGv;
 }
 
 /* ----------------------------------------------------------- integrand -- */
@@ -254,9 +322,17 @@ __host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], co
 
 /* out[k] (complex, imag 0 on the real path).  Returns false when dropped
  * (non-finite R0, feynman_integral.hpp:227-234, :254-261, :329-347). */
+/* log of the projective rescale factor (feynman_integral.hpp:201-207):
+ * scale = (psi_tr^(-D/2) (psi_tr/xi_tr)^W)^(1/(D L/2 + W)), from
+ * log psi_tr and log G (xi_tr = G psi_tr). */
+__host__ __device__ inline double log_scale(const DevParams& p, double lpsi, double lG) {
+  const double t = p.mhalfD * lpsi - (p.W != 0.0 ? p.W * lG : 0.0);
+  return p.inv_dlw * t;
+}
+
 template <class G, int K, int MODE>
-__host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E], double psi_tr,
-                                          double xi_tr, cplx (&out)[K]) {
+__host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E], double lscale,
+                                          cplx (&out)[K]) {
   constexpr int E = G::E, V = G::V, NT = V * (V + 1) / 2;
   /* vertex weights and v* (first minimum) -- feynman_integral.hpp:166-194 */
   double XV[V];
@@ -276,8 +352,7 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
       vs = v;
     }
   /* projective rescale -- :201-207 */
-  const double tgt = pow(psi_tr, p.mhalfD) * pow(psi_tr / xi_tr, p.W);
-  const double scale = pow(tgt, p.inv_dlw);
+  const double scale = exp(lscale);
   double Xi[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -308,7 +383,7 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
   const double psi = ldlt_inplace<V>(M, dinv) * xprod;
 
   if constexpr (MODE != 2) {
-    double R = p.IGtr * pow(psi, p.mhalfD);
+    double R = p.IGtr * pow_mD(p, psi);
     if (!__builtin_isfinite(R)) return false;
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = cmk(0.0, 0.0);
@@ -322,7 +397,7 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
 #pragma unroll
     for (int e = 0; e < E; e++) dot += p.m2[e] * X[e];
     const double pxi = -tr + dot;
-    R *= pow(pxi, p.mW);
+    R *= pow_mW(p, pxi);
     if (!__builtin_isfinite(R)) return false;
     out[0] = cmk(R, 0.0);
     if (K > 1) {
@@ -481,11 +556,11 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
     }
     const cplx cPsi = cxp * cdet;
     const cplx cPXi = cdot - ctr;
-    cplx cR = p.IGtr * cpow_real(cPsi, p.mhalfD);
+    cplx cR = p.IGtr * cpow_mD(p, cPsi);
     double sn, cs;
     sincos(xne, &sn, &cs);
     cR = cR * (jdet * cmk(cs, sn));
-    cR = cR * cpow_real(cPXi, p.mW);
+    cR = cR * cpow_mW(p, cPXi);
     /* Branch-free tail: the finiteness flag is returned, not branched on
      * (an early return here made the gfx950 backend at -O2/-O3 produce wrong
      * values for the 10-edge ladder under high register pressure). */
@@ -564,8 +639,8 @@ __device__ inline double wave_sum(double v) {
 
 /* Lane stream positioning: the block's first state comes from the host;
  * lane t applies the jump polynomials of the set bits of t. */
-__device__ inline Xoshiro lane_stream(const Launch& a) {
-  const uint64_t* bs = a.block_state + 4 * (size_t)blockIdx.x;
+__device__ inline Xoshiro lane_stream(const Launch& a, uint32_t bid) {
+  const uint64_t* bs = a.block_state + 4 * (size_t)bid;
   Xoshiro g{bs[0], bs[1], bs[2], bs[3]};
   const unsigned t = threadIdx.x;
   for (int j = 0; (1u << j) < blockDim.x; j++) {
@@ -581,7 +656,7 @@ __device__ inline Xoshiro lane_stream(const Launch& a) {
  * order; the block writes one row of partial moments
  * [n, drops, (s1re, s2re, s1im, s2im) x K]. */
 template <class G, int K, int MODE, int TMODE>
-__device__ inline void integrate_body(const Launch& a) {
+__device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   extern __shared__ __align__(16) char lds[];
   constexpr int E = G::E;
   constexpr int NQ = 2 + 4 * K;
@@ -589,8 +664,8 @@ __device__ inline void integrate_body(const Launch& a) {
   TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
   double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
 
-  Xoshiro rng = lane_stream(a);
-  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Xoshiro rng = lane_stream(a, bid);
+  const uint64_t lane = (uint64_t)bid * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
   const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
 
@@ -598,10 +673,10 @@ __device__ inline void integrate_body(const Launch& a) {
 #pragma unroll
   for (int q = 0; q < NQ; q++) acc[q] = 0.0;
   for (uint64_t i = i0; i < i1; i++) {
-    double X[E], psi_tr, xi_tr;
-    sample_point<G>(rng, tv, X, psi_tr, xi_tr);
+    double X[E], lpsi, lG;
+    sample_point<G>(rng, tv, X, lpsi, lG);
     cplx out[K];
-    if (!integrand<G, K, MODE>(p, X, psi_tr, xi_tr, out)) {
+    if (!integrand<G, K, MODE>(p, X, log_scale(p, lpsi, lG), out)) {
       acc[1] += 1.0;
       continue;
     }
@@ -627,7 +702,7 @@ __device__ inline void integrate_body(const Launch& a) {
   if (threadIdx.x < NQ) {
     double s = 0.0;
     for (int w = 0; w < nw; w++) s += red[w * NQ + threadIdx.x];
-    a.partials[(size_t)blockIdx.x * NQ + threadIdx.x] = s;
+    a.partials[(size_t)bid * NQ + threadIdx.x] = s;
   }
 }
 
@@ -643,7 +718,9 @@ __device__ inline void replay_body(const DevParams* prm, uint64_t n, const doubl
 #pragma unroll
   for (int e = 0; e < E; e++) X[e] = Xin[i * E + e];
   cplx r[K];
-  const bool ok = integrand<G, K, MODE>(*prm, X, psixi[2 * i], psixi[2 * i + 1], r);
+  const double ps = psixi[2 * i], xs = psixi[2 * i + 1];
+  const double ls = prm->inv_dlw * (prm->mhalfD * log(ps) + (prm->W != 0.0 ? prm->W * log(ps / xs) : 0.0));
+  const bool ok = integrand<G, K, MODE>(*prm, X, ls, r);
 #pragma unroll
   for (int k = 0; k < K; k++) {
     out[(i * K + k) * 2] = ok ? r[k].re : 0.0;
@@ -658,28 +735,47 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
   extern __shared__ __align__(16) char lds[];
   constexpr int E = G::E;
   TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
-  Xoshiro rng = lane_stream(a);
+  Xoshiro rng = lane_stream(a, blockIdx.x);
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
   const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
   for (uint64_t i = i0; i < i1; i++) {
-    double X[E], ps, xs;
-    sample_point<G>(rng, tv, X, ps, xs);
+    double X[E], lps, lG;
+    sample_point<G>(rng, tv, X, lps, lG);
 #pragma unroll
     for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
+    const double ps = exp(lps);
     pxout[2 * i] = ps;
-    pxout[2 * i + 1] = xs;
+    pxout[2 * i + 1] = exp(lG) * ps;
   }
+}
+
+/* Batched scan: every block belongs to one problem of the item list (found
+ * by binary search on block0; uniform per block). */
+template <class G, int K, int MODE, int TMODE>
+__device__ inline void batch_body(const BatchItem* items, int nitems) {
+  int lo = 0, hi = nitems - 1;
+  const uint32_t b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  integrate_body<G, K, MODE, TMODE>(items[lo].a, b - items[lo].block0);
 }
 
 }  // namespace ftr
 
 /* Instantiation macro used by the generated kernel units:
  *   FTR_INSTANTIATE(name, Graph, K, MODE, TMODE, BLOCK)
- * defines extern "C" kernels  name_int, name_rep, name_smp. */
+ * defines extern "C" kernels  name_int, name_bat, name_rep, name_smp. */
 #define FTR_INSTANTIATE(NAME, G, K, MODE, TMODE, BLOCK)                                           \
   extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_int(ftr::Launch a) {                 \
-    ftr::integrate_body<G, K, MODE, TMODE>(a);                                                    \
+    ftr::integrate_body<G, K, MODE, TMODE>(a, blockIdx.x);                                        \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_bat(const ftr::BatchItem* items,      \
+                                                                   int nitems) {                  \
+    ftr::batch_body<G, K, MODE, TMODE>(items, nitems);                                            \
   }                                                                                               \
   extern "C" __global__ void __launch_bounds__(256)                                               \
       NAME##_rep(const ftr::DevParams* prm, uint64_t n, const double* X, const double* px,        \

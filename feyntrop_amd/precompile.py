@@ -9,12 +9,12 @@ import concurrent.futures as cf
 import os
 import sys
 
-from feyntrop_amd import precompile, workloads
+from feyntrop_amd import host_path, precompile, workloads
 
 
 def problems():
     probs = dict(workloads.all_problems())
-    for k, v in workloads.scan_problems().items():
+    for k, v in workloads.scan_problems(host_path).items():
         probs[k] = v
     return probs
 

@@ -206,6 +206,84 @@ def all_problems():
     return d
 
 
-def scan_problems():
-    """Representatives of the batched-scan topologies (filled in with the scan)."""
-    return {}
+# Batched scan: 1- to 3-loop topologies with 4-10 edges and 3-7 vertices,
+# (edges, external vertices).
+SCAN_TOPOLOGIES = {
+    "box": ([(0, 1), (1, 2), (2, 3), (3, 0)], (0, 1, 2, 3)),
+    "pentagon": ([(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], (0, 1, 2, 3, 4)),
+    "kite": ([(0, 2), (0, 3), (1, 2), (1, 3), (2, 3)], (0, 1)),
+    "twoloop_3pt": ([(0, 1), (1, 3), (3, 2), (2, 0), (0, 3)], (0, 1, 2)),
+    "double_box": ([(0, 1), (2, 3), (4, 5), (0, 2), (2, 4), (1, 3), (3, 5)], (0, 1, 4, 5)),
+    "twoloop_4pt": ([(0, 1), (0, 4), (1, 5), (5, 4), (5, 2), (4, 3), (3, 2)], (0, 1, 2, 3)),
+    "penta_box": ([(0, 1), (1, 2), (2, 6), (6, 3), (3, 4), (4, 5), (5, 0), (5, 6)], (0, 1, 2, 3, 4)),
+    "wheel": ([(0, 1), (0, 2), (0, 3), (1, 2), (2, 3), (3, 1)], (0, 1, 2, 3)),
+    "threeloop_2pt": ([(0, 2), (0, 3), (2, 3), (2, 4), (3, 5), (4, 5), (4, 1), (5, 1)], (0, 1)),
+}
+
+
+def _scan_P(V, ext, minkowski, rng):
+    """Scalar products of random external momenta (4-dimensional), momentum
+    conservation on the last external vertex; Euclidean: P = -Gram (negative
+    semi-definite); Minkowski: metric (+,-,-,-)."""
+    n = len(ext)
+    mom = []
+    for _ in range(n - 1):
+        mom.append([rng.uniform(-1.5, 1.5) for _ in range(4)])
+    mom.append([-sum(m[k] for m in mom) for k in range(4)])
+    sig = (1.0, -1.0, -1.0, -1.0) if minkowski else (-1.0, -1.0, -1.0, -1.0)
+    P = [[0.0] * V for _ in range(V)]
+    for a in range(n):
+        for b in range(n):
+            P[ext[a]][ext[b]] = sum(sig[k] * mom[a][k] * mom[b][k] for k in range(4))
+    # exact zero row sums: the last row/column closes conservation
+    last = ext[-1]
+    for a in range(n - 1):
+        P[ext[a]][last] = P[last][ext[a]] = -sum(P[ext[a]][ext[b]] for b in range(n - 1))
+    P[last][last] = -sum(P[last][ext[b]] for b in range(n - 1))
+    return P
+
+
+def scan_points(n=4096, seed=0, accept=None):
+    """The batched scan: n kinematic points over SCAN_TOPOLOGIES, D in {4, 6},
+    m^2 ~ U[0,4], Euclidean (lambda = 0) or Minkowski (lambda in {0.3, 0.7}),
+    N log-uniform in [1e5, 1e7], K = 1.  Candidates are drawn from the seeded
+    generator until n of them pass `accept` (a callable prob -> bool: the
+    preprocessing's acceptance, e.g. feyntrop_amd.host_accepts)."""
+    rng = random.Random(seed)
+    names = sorted(SCAN_TOPOLOGIES)
+    out = []
+    while len(out) < n:
+        name = names[rng.randrange(len(names))]
+        edges, ext = SCAN_TOPOLOGIES[name]
+        V = max(max(e) for e in edges) + 1
+        mink = rng.random() < 0.5
+        D = rng.choice((4, 6))
+        prob = {
+            "graph": _edges(edges),
+            "dimension": D,
+            "scalarproducts": _scan_P(V, ext, mink, rng),
+            "masses_sqr": [rng.uniform(0, 4) for _ in edges],
+            "num_eps_terms": 1,
+            "lambda": rng.choice((0.3, 0.7)) if mink else 0,
+            "N": int(10 ** rng.uniform(5, 7)),
+            "seed": rng.randrange(2 ** 32),
+            "_topology": name,
+        }
+        if accept is None or accept(prob):
+            out.append(prob)
+    return out
+
+
+def scan_problems(path_of=None):
+    """One representative per (topology, kernel path) of the scan, so that
+    their kernels can be specialised ahead of time.  path_of(prob) returns the
+    path (0, 1, 2) or None when the preprocessing rejects the point."""
+    reps = {}
+    for prob in scan_points(1024, seed=12345):
+        path = path_of(prob) if path_of else (2 if prob["lambda"] else 0)
+        if path is None:
+            continue
+        key = (prob["_topology"], path)
+        if key not in reps:
+            reps[key] = prob
+    return {f"scan_{k[0]}_path{k[1]}": v for k, v in reps.items()}

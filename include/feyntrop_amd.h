@@ -94,7 +94,10 @@ int ftr_create(int device, ftr_ctx** ctx);
 void ftr_destroy(ftr_ctx* ctx);
 const char* ftr_last_error(const ftr_ctx* ctx);
 
-/* Validate + preprocess + specialise kernels.  *out owned by the caller. */
+/* Validate + preprocess + specialise kernels.  *out owned by the caller.
+ * ctx may be NULL: host-only preparation (validation, regime, J-table) on
+ * any host; such a problem can be inspected but not integrated.  A problem
+ * must be freed before its context is destroyed. */
 int ftr_prepare(ftr_ctx* ctx, const ftr_input* in, ftr_problem** out);
 void ftr_problem_free(ftr_problem* p);
 int ftr_problem_info(const ftr_problem* p, ftr_info* info);
@@ -128,6 +131,17 @@ void ftr_free_string(char* s);
 /* Batched scan: n problems, one launch of a persistent kernel per batch. */
 int ftr_integrate_batch(ftr_ctx* ctx, int n, ftr_problem* const* probs, const uint64_t* N,
                         const uint64_t* seeds, ftr_moments* out);
+
+/* Autotuner (no reference counterpart; replaces the reference's fixed OpenMP
+ * chunking): times every feasible (J-table placement, block size) kernel
+ * variant on n_probe samples, keeps the fastest for the problem and records
+ * it in the cache directory, where later ftr_prepare calls pick it up.
+ * table_mode: 0 whole J-table in LDS, 1 J/r in LDS + rest from L2/HBM,
+ * 2 all from L2/HBM. */
+int ftr_autotune(ftr_ctx* ctx, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block,
+                 double* best_kernel_ms);
+int ftr_set_variant(ftr_ctx* ctx, ftr_problem* p, int table_mode, int block);
+int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block);
 
 /* Kernel code-object cache directory (default: the package's kcache/). */
 int ftr_set_cache_dir(const char* dir);

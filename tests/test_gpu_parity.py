@@ -226,3 +226,41 @@ def test_cli_readme_wheel(golden_fixtures):
         assert abs(a - b) <= 3 * np.hypot(da, db)
         assert d["integral"][k][1] == [0.0, 0.0]
     assert "(Effective) kinematic regime: Euclidean (generic)." in out.stderr
+
+
+def test_batch_equals_individual(engine):
+    """The batched scan draws the same samples as one call per problem: the
+    same counts, sums equal up to summation order."""
+    import feyntrop_amd
+    pts = W.scan_points(24, seed=7, accept=feyntrop_amd.host_accepts)
+    probs = [engine.prepare(p) for p in pts]
+    Ns = [min(p["N"], 400_000) for p in pts]
+    seeds = [p["seed"] for p in pts]
+    print(f"[batch] {len(pts)} points, paths {sorted(set(q.info.path for q in probs))}", flush=True)
+    outs = engine.integrate_batch(probs, Ns, seeds)
+    for q, N, s, mb in zip(probs, Ns, seeds, outs):
+        mi = engine.integrate(q, N, s)
+        nb, s1b, s2b = mb.arrays()
+        ni, s1i, s2i = mi.arrays()
+        assert np.array_equal(nb, ni)
+        assert mb.dropped == mi.dropped
+        assert np.allclose(s1b, s1i, rtol=1e-10, atol=1e-12 * np.abs(s1i).max())
+        assert np.allclose(s2b, s2i, rtol=1e-10, atol=0)
+
+
+def test_autotune_keeps_results(engine, tmp_path):
+    """Every kernel variant the autotuner may pick draws the same samples."""
+    import feyntrop_amd
+    feyntrop_amd.lib().ftr_set_cache_dir(str(tmp_path).encode())
+    try:
+        prob = W.cubic5(N=400_000)
+        p = engine.prepare(prob)
+        ref = engine.integrate(p, prob["N"], 2)
+        tm, blk, ms = engine.autotune(p, n_probe=400_000)
+        print(f"[autotune] cubic5 -> table_mode {tm} block {blk} ({ms:.3f} ms)", flush=True)
+        assert engine.variant(p) == (tm, blk)
+        m = engine.integrate(p, prob["N"], 2)
+        assert np.array_equal(m.arrays()[0], ref.arrays()[0])
+        assert np.allclose(m.arrays()[1], ref.arrays()[1], rtol=1e-11)
+    finally:
+        feyntrop_amd.lib().ftr_set_cache_dir(None)
