@@ -96,6 +96,17 @@ def lib():
     L.ftr_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
     L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
+    L.ftr_precompile_learned.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
+    L.ftr_learner_create.argtypes = [vp, vp, C.c_uint64, C.POINTER(vp)]
+    L.ftr_learner_free.argtypes = [vp]
+    L.ftr_learner_params.argtypes = [vp, vp, vp]
+    L.ftr_learner_forward.argtypes = [vp, vp, vp]
+    L.ftr_learner_backward.argtypes = [vp, vp, vp, vp]
+    L.ftr_learner_train.argtypes = [vp, vp, C.c_int, C.c_uint64, C.c_uint64, C.c_double, vp]
+    L.ftr_learner_tables.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.ftr_learner_integrate.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts),
+                                        C.POINTER(ftr_moments)]
+    L.ftr_learner_sample.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp, vp]
     _lib = L
     return L
 
@@ -129,6 +140,15 @@ def precompile(prob):
     inp = _Input(prob)
     err = C.create_string_buffer(8192)
     rc = lib().ftr_precompile(C.byref(inp.s), err, 8192)
+    if rc:
+        raise FeyntropError(rc, err.value.decode(errors="replace"))
+
+
+def precompile_learned(prob):
+    """Specialise and cache the learned-proposal kernels of a problem."""
+    inp = _Input(prob)
+    err = C.create_string_buffer(8192)
+    rc = lib().ftr_precompile_learned(C.byref(inp.s), err, 8192)
     if rc:
         raise FeyntropError(rc, err.value.decode(errors="replace"))
 
@@ -272,6 +292,88 @@ class Engine:
         lib().ftr_free_string(out)
         lib().ftr_free_string(log)
         return rc, o, lg
+
+
+LEARN_HIDDEN = 128
+LEARN_NPARAMS = 128 * 32 + 128 + 2 * (128 * 128 + 128) + 32 * 128 + 32
+
+
+class Learner:
+    """Learned proposal of one prepared problem (include/feyntrop_amd.h,
+    ftr_learner_*).  Parameters: FP32 vector of LEARN_NPARAMS values
+    (W1[128x32] b1 W2[128x128] b2 W3[128x128] b3 W4[32x128] b4)."""
+
+    def __init__(self, engine, p, init_seed=0):
+        self.engine, self.p = engine, p
+        h = C.c_void_p()
+        rc = lib().ftr_learner_create(engine.h, p.h, int(init_seed), C.byref(h))
+        if rc:
+            raise engine._err(rc)
+        self.h = h
+        self.E = p.E
+        self.rows = max(32, 1 << p.E)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ftr_learner_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise self.engine._err(rc)
+
+    def params(self):
+        out = np.zeros(LEARN_NPARAMS, np.float32)
+        self._chk(lib().ftr_learner_params(self.h, out.ctypes.data, None))
+        return out
+
+    def set_params(self, par):
+        par = np.ascontiguousarray(par, dtype=np.float32)
+        assert par.size == LEARN_NPARAMS
+        self._chk(lib().ftr_learner_params(self.h, None, par.ctypes.data))
+
+    def forward(self):
+        Y = np.zeros((self.rows, 32), np.float32)
+        self._chk(lib().ftr_learner_forward(self.engine.h, self.h, Y.ctypes.data))
+        return Y
+
+    def backward(self, dY):
+        dY = np.ascontiguousarray(dY, dtype=np.float32)
+        assert dY.shape == (self.rows, 32)
+        g = np.zeros(LEARN_NPARAMS, np.float32)
+        self._chk(lib().ftr_learner_backward(self.engine.h, self.h, dY.ctypes.data, g.ctypes.data))
+        return g
+
+    def train(self, steps, samples, seed=1, lr=3e-3):
+        """Returns per step (relative second moment of the eps^0 real part, kernel ms)."""
+        st = np.zeros((max(steps, 1), 2))
+        self._chk(lib().ftr_learner_train(self.engine.h, self.h, int(steps), int(samples), int(seed),
+                                          float(lr), st.ctypes.data))
+        return st[:steps]
+
+    def tables(self):
+        n, E = 1 << self.E, self.E
+        lw, lratio = np.zeros((n, E)), np.zeros((n, E))
+        lZ, lia, lrad, ldiff = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        self._chk(lib().ftr_learner_tables(self.engine.h, self.h, lw.ctypes.data, lZ.ctypes.data,
+                                           lratio.ctypes.data, lia.ctypes.data, lrad.ctypes.data,
+                                           ldiff.ctypes.data))
+        return dict(lw=lw, lZ=lZ, lratio=lratio, lia=lia, lrad=lrad, ldiff=ldiff)
+
+    def integrate(self, N, seed=0, lanes=0):
+        m = ftr_moments()
+        opts = ftr_opts(0, -1, int(lanes))
+        self._chk(lib().ftr_learner_integrate(self.engine.h, self.h, int(N), int(seed), C.byref(opts),
+                                              C.byref(m)))
+        return m
+
+    def sample(self, first, n, seed, X_ptr, w_ptr, path_ptr):
+        """Device pointers of n*E doubles, n doubles and n uint64."""
+        self._chk(lib().ftr_learner_sample(self.engine.h, self.h, int(first), int(n), int(seed),
+                                           C.c_void_p(X_ptr), C.c_void_p(w_ptr), C.c_void_p(path_ptr)))
 
 
 def integral(engine, prob):

@@ -1,5 +1,6 @@
-"""Build step: embed the device header (ftr_abi.hpp + ftr_device.hpp) into the
-runtime library as one C string, the source hipRTC specialises per graph."""
+"""Build step: embed the device headers into the runtime library as C strings:
+ftr_abi.hpp + ftr_device.hpp (the source hipRTC specialises per graph) and
+ftr_mlp.hpp (the learned proposal's network, compiled once)."""
 import os
 import sys
 
@@ -10,13 +11,22 @@ dev = dev.replace('#include "ftr_abi.hpp"', "")
 src = (abi + "\n" + dev).replace("#pragma once", "")
 delim = "FTRSRC"
 assert (")" + delim) not in src
+mlp = open(os.path.join(here, "kernels", "ftr_mlp.hpp")).read().replace("#pragma once", "")
+assert (")" + delim) not in mlp
 out = sys.argv[1]
+
+
+def literal(f, name, text):
+    # split into chunks: some compilers limit single literal length
+    f.write("extern const char* %s;\n" % name)
+    f.write("static const char %s_text[] =\n" % name)
+    for i in range(0, len(text), 8000):
+        f.write('R"%s(%s)%s"\n' % (delim, text[i:i + 8000], delim))
+    f.write(";\nconst char* %s = %s_text;\n" % (name, name))
+
+
 with open(out + ".tmp", "w") as f:
     f.write("/* generated by embed_source.py -- do not edit */\n")
-    # split into chunks: some compilers limit single literal length
-    f.write("extern const char* ftr_embedded_device_source;\n")
-    f.write("static const char ftr_src_text[] =\n")
-    for i in range(0, len(src), 8000):
-        f.write('R"%s(%s)%s"\n' % (delim, src[i:i + 8000], delim))
-    f.write(";\nconst char* ftr_embedded_device_source = ftr_src_text;\n")
+    literal(f, "ftr_embedded_device_source", src)
+    literal(f, "ftr_embedded_mlp_source", mlp)
 os.replace(out + ".tmp", out)

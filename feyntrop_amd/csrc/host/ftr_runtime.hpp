@@ -1,0 +1,92 @@
+/*
+ * ftr_runtime.hpp -- internals shared by the runtime's translation units
+ * (runtime.cpp: problems and plain sampling; learn.cpp: learned proposal).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../include/feyntrop_amd.h"
+#include "../kernels/ftr_abi.hpp"
+#include "ftr_host.hpp"
+
+extern const char* ftr_embedded_device_source;
+extern const char* ftr_embedded_mlp_source;
+
+using ftrh::Error;
+
+struct Variant {
+  int mode = 0, tmode = 0, block = 256, K = 1;
+};
+
+struct Module {
+  hipModule_t mod = nullptr;
+  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr, f_bat = nullptr;
+};
+
+struct ftr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop;
+  std::string err;
+  std::map<std::string, Module> modules;
+  std::map<std::string, hipModule_t> raw_modules; /* every loaded code object */
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  /* growable device scratch: block states, lane polys, partials */
+  void* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  std::vector<double> h_partials;
+  /* batched scans */
+  void* d_batch = nullptr;
+  size_t batch_bytes = 0;
+  double* h_pinned = nullptr; /* pinned copy-back buffer of the batch graph */
+  size_t pinned_bytes = 0;
+};
+
+struct ftr_problem {
+  ftrh::Prep prep;
+  ftr_ctx* ctx = nullptr;
+  int device = 0;
+  Variant var;
+  size_t lds = 0;
+  Module m;
+  ftr::DevParams hprm;
+  ftr::DevParams* d_prm = nullptr;
+  double *d_w = nullptr, *d_jr = nullptr, *d_ir = nullptr;
+  uint8_t* d_mml = nullptr;
+};
+
+/* ------------------------------------------------------------ helpers -- */
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) throw Error(FTR_ERR_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+/* Geometry + stream positioning for a run of n samples starting at sample
+ * `first` of the stream seeded with `seed`.  Writes block states and lane
+ * polynomials to the context scratch. */
+struct Geometry {
+  uint64_t nblocks = 0, S = 0, lanes = 0;
+  uint64_t* d_states = nullptr;
+  uint64_t* d_polys = nullptr;
+  double* d_partials = nullptr;
+};
+
+/* kernel source of a problem's variant; `learned` adds the learned-proposal
+ * kernels */
+std::string full_source(const ftrh::Prep& pr, const Variant& v, bool learned = false);
+/* code object for a source (hipRTC, cached on disk) */
+std::string code_object(const std::string& src, std::string* path_out);
+/* load (once per context) and resolve the kernels named `names` */
+hipModule_t load_module_raw(ftr_ctx* c, const std::string& src);
+Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
+                      int nq, hipFunction_t occ_fn);
+void ensure_scratch(ftr_ctx* c, size_t bytes);
+void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
+Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit);
+int fail(ftr_ctx* c, int code, const std::string& msg);
+extern thread_local std::string g_host_err;

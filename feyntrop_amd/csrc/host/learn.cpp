@@ -1,0 +1,569 @@
+/*
+ * learn.cpp -- learned proposal for the tropical sampler (no reference
+ * counterpart; it sits on top of the sampler of feynman_integral.hpp:31-77).
+ *
+ * Per problem: an MLP over the 2^E subgraph states (one row per state) whose
+ * output tilts the cut weights and shifts the radial exponents of the
+ * tropical proposal.  The network runs on the MFMA units in BF16 (module
+ * ftr_mlp.hpp, compiled once); the tables it produces are FP64, and so is the
+ * importance weight p_trop/q of every sample, so the estimate is unbiased
+ * whatever the network outputs.  Training: forward KL between |f| p_trop and
+ * q, estimated from proposal samples (self-normalised), Adam.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ftr_runtime.hpp"
+
+namespace {
+
+constexpr int H = FTR_LEARN_HIDDEN;
+constexpr int NIN = 32, NOUT = 32;
+constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + (size_t)H * NIN;
+constexpr size_t OFF_W2 = OFF_B1 + H, OFF_B2 = OFF_W2 + (size_t)H * H;
+constexpr size_t OFF_W3 = OFF_B2 + H, OFF_B3 = OFF_W3 + (size_t)H * H;
+constexpr size_t OFF_W4 = OFF_B3 + H, OFF_B4 = OFF_W4 + (size_t)NOUT * H;
+constexpr size_t NPAR = OFF_B4 + NOUT;
+static_assert(NPAR == FTR_LEARN_NPARAMS, "parameter layout");
+constexpr int MAX_E = 16;          /* cut order packed 4 bits per step */
+constexpr double DMAX = 8.0;       /* |Delta| bound (cut-weight tilt) */
+constexpr double SMAX = 0.6;       /* |sigma| bound: a = r e^sigma < 2r keeps p/q of finite variance */
+
+struct MlpModule {
+  hipModule_t mod = nullptr;
+  hipFunction_t gbb, gfb, silu_bwd, colsum, adam, fill, fill_f, to_bf16, tables, tgrad;
+};
+
+MlpModule load_mlp(ftr_ctx* c) {
+  MlpModule m;
+  m.mod = load_module_raw(c, ftr_embedded_mlp_source);
+  HIPCHK(hipModuleGetFunction(&m.gbb, m.mod, "mlp_gemm_bb"));
+  HIPCHK(hipModuleGetFunction(&m.gfb, m.mod, "mlp_gemm_fb"));
+  HIPCHK(hipModuleGetFunction(&m.silu_bwd, m.mod, "mlp_silu_bwd"));
+  HIPCHK(hipModuleGetFunction(&m.colsum, m.mod, "mlp_colsum"));
+  HIPCHK(hipModuleGetFunction(&m.adam, m.mod, "mlp_adam"));
+  HIPCHK(hipModuleGetFunction(&m.fill, m.mod, "mlp_fill"));
+  HIPCHK(hipModuleGetFunction(&m.fill_f, m.mod, "mlp_fill_f"));
+  HIPCHK(hipModuleGetFunction(&m.to_bf16, m.mod, "mlp_to_bf16"));
+  HIPCHK(hipModuleGetFunction(&m.tables, m.mod, "mlp_build_tables"));
+  HIPCHK(hipModuleGetFunction(&m.tgrad, m.mod, "mlp_table_grad"));
+  return m;
+}
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct ftr_learner {
+  ftr_problem* p = nullptr;
+  int device = 0;
+  Variant var; /* the problem's kernel variant when the learner was made */
+  size_t lds = 0;
+  int E = 0;
+  uint32_t NT = 0;
+  int M = 0; /* network rows: max(32, 2^E) */
+  MlpModule mlp;
+  hipFunction_t f_lev = nullptr, f_lgr = nullptr, f_lsm = nullptr;
+  float *d_par = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
+  uint16_t *d_parb = nullptr, *d_X0b = nullptr, *d_H1 = nullptr, *d_H2 = nullptr, *d_H3 = nullptr;
+  float *d_Z1 = nullptr, *d_Z2 = nullptr, *d_Z3 = nullptr, *d_Y = nullptr, *d_dY = nullptr, *d_dH = nullptr,
+        *d_dZ = nullptr;
+  double *d_lw = nullptr, *d_lZ = nullptr, *d_lratio = nullptr, *d_lia = nullptr, *d_lrad = nullptr,
+         *d_ldiff = nullptr, *d_r = nullptr, *d_gC = nullptr, *d_gS = nullptr, *d_gsum = nullptr;
+  int adam_t = 0;
+  bool tables_valid = false;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+template <class T>
+T* dalloc(ftr_learner* l, size_t n) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 256)));
+  l->allocs.push_back(p);
+  return (T*)p;
+}
+
+void launch(hipFunction_t f, unsigned gx, unsigned gy, unsigned block, hipStream_t s, void** args) {
+  HIPCHK(hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, 0, s, args, nullptr));
+}
+
+/* C = act(A B + bias), operands addressed by strides (see ftr_mlp.hpp) */
+void gemm(ftr_ctx* c, ftr_learner* l, bool a_float, const void* A, uint64_t sai, uint64_t sak, const uint16_t* B,
+          uint64_t sbk, uint64_t sbj, const float* bias, float* C, uint64_t ldc, uint16_t* Cb, uint64_t ldcb, int M,
+          int N, int K, int act) {
+  if (M % 32 || N % 32 || K % 8) throw Error(FTR_ERR_INVALID, "network GEMM shape not tiled");
+  /* long reductions (weight gradients over all states) split K over blocks
+   * that add into C atomically; C must be zero then */
+  int splits = 1, kc = K;
+  if (K >= 1024 && !Cb && !bias) {
+    kc = 256;
+    splits = (K + kc - 1) / kc;
+  }
+  void* args[] = {(void*)&A, &sai, &sak, (void*)&B, &sbk, &sbj, (void*)&bias, &C, &ldc, &Cb, &ldcb, &K, &act, &kc};
+  HIPCHK(hipModuleLaunchKernel(a_float ? l->mlp.gfb : l->mlp.gbb, (unsigned)(N / 32), (unsigned)(M / 32),
+                               (unsigned)splits, 64, 1, 1, 0, c->stream, args, nullptr));
+}
+
+void fill_d(ftr_ctx* c, ftr_learner* l, double* p, uint64_t n, double v) {
+  void* args[] = {&p, &n, &v};
+  launch(l->mlp.fill, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
+}
+
+void fill_f(ftr_ctx* c, ftr_learner* l, float* p, uint64_t n, float v) {
+  void* args[] = {&p, &n, &v};
+  launch(l->mlp.fill_f, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
+}
+
+void refresh_bf16(ftr_ctx* c, ftr_learner* l) {
+  uint64_t n = NPAR;
+  void* args[] = {&l->d_par, &l->d_parb, &n};
+  launch(l->mlp.to_bf16, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
+}
+
+void forward(ftr_ctx* c, ftr_learner* l) {
+  const uint16_t* W = l->d_parb;
+  const float* P = l->d_par;
+  const int M = l->M;
+  /* Z = X W^T + b: B[k][j] = W[j][k] */
+  gemm(c, l, false, l->d_X0b, NIN, 1, W + OFF_W1, 1, NIN, P + OFF_B1, l->d_Z1, H, l->d_H1, H, M, H, NIN, 1);
+  gemm(c, l, false, l->d_H1, H, 1, W + OFF_W2, 1, H, P + OFF_B2, l->d_Z2, H, l->d_H2, H, M, H, H, 1);
+  gemm(c, l, false, l->d_H2, H, 1, W + OFF_W3, 1, H, P + OFF_B3, l->d_Z3, H, l->d_H3, H, M, H, H, 1);
+  gemm(c, l, false, l->d_H3, H, 1, W + OFF_W4, 1, H, P + OFF_B4, l->d_Y, NOUT, nullptr, 0, M, NOUT, H, 0);
+}
+
+/* db += column sums of dZ (db zeroed beforehand) */
+void colsum(ftr_ctx* c, ftr_learner* l, const float* dZ, float* db, int rows, int cols) {
+  int chunk = 256;
+  void* args[] = {(void*)&dZ, &db, &rows, &cols, &chunk};
+  launch(l->mlp.colsum, (unsigned)((cols + 63) / 64), (unsigned)((rows + chunk - 1) / chunk), 64, c->stream, args);
+}
+
+void silu_bwd(ftr_ctx* c, ftr_learner* l, const float* dH, const float* Z, float* dZ, uint64_t n) {
+  void* args[] = {(void*)&dH, (void*)&Z, &dZ, &n};
+  launch(l->mlp.silu_bwd, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
+}
+
+/* parameter gradient of sum(dY * Y) into d_grad (forward must have run) */
+void backward(ftr_ctx* c, ftr_learner* l) {
+  const uint16_t* W = l->d_parb;
+  float* G = l->d_grad;
+  const int M = l->M;
+  fill_f(c, l, G, NPAR, 0.0f); /* split-K GEMMs and column sums accumulate */
+  /* layer 4: dW4 = dY^T H3, db4 = colsum dY, dH3 = dY W4 */
+  gemm(c, l, true, l->d_dY, 1, NOUT, l->d_H3, H, 1, nullptr, G + OFF_W4, H, nullptr, 0, NOUT, H, M, 0);
+  colsum(c, l, l->d_dY, G + OFF_B4, M, NOUT);
+  gemm(c, l, true, l->d_dY, NOUT, 1, W + OFF_W4, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, NOUT, 0);
+  const uint64_t n = (uint64_t)M * H;
+  /* layer 3 */
+  silu_bwd(c, l, l->d_dH, l->d_Z3, l->d_dZ, n);
+  gemm(c, l, true, l->d_dZ, 1, H, l->d_H2, H, 1, nullptr, G + OFF_W3, H, nullptr, 0, H, H, M, 0);
+  colsum(c, l, l->d_dZ, G + OFF_B3, M, H);
+  gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W3, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 0);
+  /* layer 2 */
+  silu_bwd(c, l, l->d_dH, l->d_Z2, l->d_dZ, n);
+  gemm(c, l, true, l->d_dZ, 1, H, l->d_H1, H, 1, nullptr, G + OFF_W2, H, nullptr, 0, H, H, M, 0);
+  colsum(c, l, l->d_dZ, G + OFF_B2, M, H);
+  gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W2, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 0);
+  /* layer 1 */
+  silu_bwd(c, l, l->d_dH, l->d_Z1, l->d_dZ, n);
+  gemm(c, l, true, l->d_dZ, 1, H, l->d_X0b, NIN, 1, nullptr, G + OFF_W1, NIN, nullptr, 0, H, NIN, M, 0);
+  colsum(c, l, l->d_dZ, G + OFF_B1, M, H);
+}
+
+void build_tables(ftr_ctx* c, ftr_learner* l) {
+  int E = l->E;
+  double dmax = DMAX, smax = SMAX;
+  void* args[] = {&l->d_Y, &E, &l->p->d_w, &l->p->d_jr, &l->d_r, &l->d_lw, &l->d_lZ, &l->d_lratio,
+                  &l->d_lia, &l->d_lrad, &l->d_ldiff, &dmax, &smax};
+  launch(l->mlp.tables, (l->NT + 255) / 256, 1, 256, c->stream, args);
+}
+
+void ensure_tables(ftr_ctx* c, ftr_learner* l) {
+  if (l->tables_valid) return;
+  forward(c, l);
+  build_tables(c, l);
+  l->tables_valid = true;
+}
+
+ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
+  ftr_problem* p = l->p;
+  ftr::LearnLaunch L;
+  memset(&L, 0, sizeof L);
+  L.a.prm = p->d_prm;
+  L.a.t_w = p->d_w;
+  L.a.t_jr = p->d_jr;
+  L.a.t_ir = p->d_ir;
+  L.a.t_mmL = p->d_mml;
+  L.a.block_state = G.d_states;
+  L.a.lane_poly = G.d_polys;
+  L.a.N = N;
+  L.a.S = G.S;
+  L.a.partials = G.d_partials;
+  L.lw = l->d_lw;
+  L.lZ = l->d_lZ;
+  L.lratio = l->d_lratio;
+  L.lia = l->d_lia;
+  L.lrad = l->d_lrad;
+  L.ldiff = l->d_ldiff;
+  L.gC = l->d_gC;
+  L.gS = l->d_gS;
+  L.gsum = l->d_gsum;
+  return L;
+}
+
+/* moments of the block partials (host, block order) */
+void reduce_partials(ftr_ctx* c, const Geometry& G, int K, int path, ftr_moments* out) {
+  const int NQ = 2 + 4 * K;
+  c->h_partials.resize((size_t)G.nblocks * NQ);
+  HIPCHK(hipMemcpyDtoHAsync(c->h_partials.data(), G.d_partials, c->h_partials.size() * 8, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> tot(NQ, 0.0);
+  for (uint64_t b = 0; b < G.nblocks; b++)
+    for (int q = 0; q < NQ; q++) tot[q] += c->h_partials[b * NQ + q];
+  out->K = K;
+  out->dropped = (uint64_t)tot[1];
+  out->lanes = G.lanes;
+  out->samples_per_lane = G.S;
+  for (int k = 0; k < K; k++) {
+    out->n[k][0] = out->n[k][1] = tot[0];
+    out->s1[k][0] = tot[2 + 4 * k];
+    out->s2[k][0] = tot[3 + 4 * k];
+    out->s1[k][1] = tot[4 + 4 * k];
+    out->s2[k][1] = tot[5 + 4 * k];
+    if (path != 2) {
+      out->n[k][1] = 2;
+      out->s1[k][1] = 0;
+      out->s2[k][1] = 0;
+    }
+  }
+}
+
+int fail_exc(ftr_ctx* c, std::exception& e) {
+  const Error* fe = dynamic_cast<const Error*>(&e);
+  return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learner** out) {
+  if (!out) return FTR_ERR_INVALID;
+  *out = nullptr;
+  if (!c || !p || !p->m.f_int) return fail(c, FTR_ERR_INVALID, "learner needs a problem prepared on a device");
+  ftr_learner* l = new ftr_learner();
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    const int E = p->prep.g.E;
+    if (E > MAX_E) throw Error(FTR_ERR_LIMIT, "the learned proposal supports at most 16 edges");
+    l->p = p;
+    l->device = c->device;
+    l->E = E;
+    l->NT = 1u << E;
+    l->M = std::max<int>(32, (int)l->NT);
+    l->mlp = load_mlp(c);
+    l->var = p->var;
+    l->lds = p->lds;
+    const hipModule_t gm = load_module_raw(c, full_source(p->prep, l->var, true));
+    HIPCHK(hipModuleGetFunction(&l->f_lev, gm, "ftrl_lev"));
+    HIPCHK(hipModuleGetFunction(&l->f_lgr, gm, "ftrl_lgr"));
+    HIPCHK(hipModuleGetFunction(&l->f_lsm, gm, "ftrl_lsm"));
+    if (l->lds > 64 * 1024)
+      for (hipFunction_t f : {l->f_lev, l->f_lgr, l->f_lsm})
+        (void)hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l->lds);
+    const size_t M = l->M, NT = l->NT;
+    l->d_par = dalloc<float>(l, NPAR);
+    l->d_grad = dalloc<float>(l, NPAR);
+    l->d_m = dalloc<float>(l, NPAR);
+    l->d_v = dalloc<float>(l, NPAR);
+    l->d_parb = dalloc<uint16_t>(l, NPAR);
+    l->d_X0b = dalloc<uint16_t>(l, M * NIN);
+    l->d_H1 = dalloc<uint16_t>(l, M * H);
+    l->d_H2 = dalloc<uint16_t>(l, M * H);
+    l->d_H3 = dalloc<uint16_t>(l, M * H);
+    l->d_Z1 = dalloc<float>(l, M * H);
+    l->d_Z2 = dalloc<float>(l, M * H);
+    l->d_Z3 = dalloc<float>(l, M * H);
+    l->d_dH = dalloc<float>(l, M * H);
+    l->d_dZ = dalloc<float>(l, M * H);
+    l->d_Y = dalloc<float>(l, M * NOUT);
+    l->d_dY = dalloc<float>(l, M * NOUT);
+    l->d_lw = dalloc<double>(l, NT * E);
+    l->d_lratio = dalloc<double>(l, NT * E);
+    l->d_gC = dalloc<double>(l, NT * E);
+    l->d_lZ = dalloc<double>(l, NT);
+    l->d_lia = dalloc<double>(l, NT);
+    l->d_lrad = dalloc<double>(l, NT);
+    l->d_ldiff = dalloc<double>(l, NT);
+    l->d_r = dalloc<double>(l, NT);
+    l->d_gS = dalloc<double>(l, NT);
+    l->d_gsum = dalloc<double>(l, 2);
+    /* initial parameters: hidden layers uniform(+-sqrt(3/fan_in)), zero
+     * biases, zero output layer (q = p_trop) */
+    std::vector<float> par(NPAR, 0.0f);
+    uint64_t s = init_seed;
+    auto init = [&](size_t off, int rows, int cols) {
+      const double a = std::sqrt(3.0 / cols);
+      for (size_t i = 0; i < (size_t)rows * cols; i++)
+        par[off + i] = (float)(a * (2.0 * ((splitmix(s) >> 11) * 0x1.0p-53) - 1.0));
+    };
+    init(OFF_W1, H, NIN);
+    init(OFF_W2, H, H);
+    init(OFF_W3, H, H);
+    /* inputs per state g: its edge bits, popcount/E, 1, r(g)/E */
+    std::vector<float> X0((size_t)M * NIN, 0.0f);
+    for (uint32_t g = 0; g < l->NT; g++) {
+      float* x = &X0[(size_t)g * NIN];
+      for (int e = 0; e < E; e++) x[e] = (float)((g >> e) & 1u);
+      x[MAX_E] = (float)__builtin_popcount(g) / (float)E;
+      x[MAX_E + 1] = 1.0f;
+      x[MAX_E + 2] = (float)(p->prep.r[g] / E);
+    }
+    std::vector<float> zeros(NPAR, 0.0f);
+    float* d_x0f = nullptr;
+    HIPCHK(hipMalloc((void**)&d_x0f, X0.size() * 4));
+    HIPCHK(hipMemcpyHtoDAsync(d_x0f, X0.data(), X0.size() * 4, c->stream));
+    {
+      uint64_t n = X0.size();
+      void* args[] = {&d_x0f, &l->d_X0b, &n};
+      launch(l->mlp.to_bf16, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
+    }
+    HIPCHK(hipMemcpyHtoDAsync(l->d_par, par.data(), NPAR * 4, c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(l->d_m, zeros.data(), NPAR * 4, c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(l->d_v, zeros.data(), NPAR * 4, c->stream));
+    HIPCHK(hipMemcpyHtoDAsync(l->d_r, p->prep.r.data(), NT * 8, c->stream));
+    refresh_bf16(c, l);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_x0f);
+  } catch (std::exception& e) {
+    ftr_learner_free(l);
+    return fail_exc(c, e);
+  }
+  *out = l;
+  return FTR_OK;
+}
+
+void ftr_learner_free(ftr_learner* l) {
+  if (!l) return;
+  (void)hipSetDevice(l->device);
+  for (void* p : l->allocs) (void)hipFree(p);
+  delete l;
+}
+
+int ftr_learner_params(ftr_learner* l, float* get, const float* set) {
+  if (!l) return FTR_ERR_INVALID;
+  ftr_ctx* c = l->p->ctx;
+  try {
+    HIPCHK(hipSetDevice(l->device));
+    if (get) {
+      HIPCHK(hipMemcpyDtoHAsync(get, l->d_par, NPAR * 4, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (set) {
+      HIPCHK(hipMemcpyHtoDAsync(l->d_par, set, NPAR * 4, c->stream));
+      refresh_bf16(c, l);
+      HIPCHK(hipStreamSynchronize(c->stream));
+      l->tables_valid = false;
+    }
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_forward(ftr_ctx* c, ftr_learner* l, float* Y) {
+  if (!c || !l || !Y) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    forward(c, l);
+    build_tables(c, l);
+    l->tables_valid = true;
+    HIPCHK(hipMemcpyDtoHAsync(Y, l->d_Y, (size_t)l->M * NOUT * 4, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_backward(ftr_ctx* c, ftr_learner* l, const float* dY, float* grad) {
+  if (!c || !l || !dY || !grad) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    forward(c, l);
+    HIPCHK(hipMemcpyHtoDAsync(l->d_dY, dY, (size_t)l->M * NOUT * 4, c->stream));
+    backward(c, l);
+    HIPCHK(hipMemcpyDtoHAsync(grad, l->d_grad, NPAR * 4, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_train(ftr_ctx* c, ftr_learner* l, int steps, uint64_t samples, uint64_t seed, double lr,
+                      double* stats) {
+  if (!c || !l || steps < 0 || samples == 0 || !(lr > 0)) return FTR_ERR_INVALID;
+  ftr_problem* p = l->p;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    const int K = p->prep.K, NQ = 2 + 4 * K;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int st = 0; st < steps; st++) {
+      forward(c, l);
+      build_tables(c, l);
+      fill_d(c, l, l->d_gC, (uint64_t)l->NT * l->E, 0.0);
+      fill_d(c, l, l->d_gS, l->NT, 0.0);
+      fill_d(c, l, l->d_gsum, 2, 0.0);
+      Geometry G = setup_stream(p, 0, samples, seed + (uint64_t)st, 0, l->var.block, NQ, l->f_lgr);
+      ftr::LearnLaunch L = learn_launch(l, G, samples);
+      void* args[] = {&L};
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      HIPCHK(hipModuleLaunchKernel(l->f_lgr, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
+                                   c->stream, args, nullptr));
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      fill_f(c, l, l->d_dY, (uint64_t)l->M * NOUT, 0.0f);
+      {
+        int E = l->E;
+        double dmax = DMAX, smax = SMAX;
+        void* targs[] = {&l->d_Y, &E, &l->d_lw, &l->d_lZ, &l->d_gC, &l->d_gS, &l->d_gsum, &l->d_dY, &dmax, &smax};
+        launch(l->mlp.tgrad, (l->NT + 255) / 256, 1, 256, c->stream, targs);
+      }
+      backward(c, l);
+      l->adam_t++;
+      const float c1 = 1.0f - std::pow(b1, (float)l->adam_t), c2 = 1.0f - std::pow(b2, (float)l->adam_t);
+      {
+        uint64_t n = NPAR;
+        float flr = (float)lr;
+        void* aargs[] = {&l->d_par, &l->d_parb, &l->d_grad, &l->d_m, &l->d_v, &n, &flr,
+                         (void*)&b1, (void*)&b2, (void*)&eps, (void*)&c1, (void*)&c2};
+        launch(l->mlp.adam, (unsigned)((n + 255) / 256), 1, 256, c->stream, aargs);
+      }
+      if (stats) {
+        ftr_moments m;
+        memset(&m, 0, sizeof m);
+        reduce_partials(c, G, K, p->prep.path, &m);
+        const double n = m.n[0][0], s1 = m.s1[0][0], s2 = m.s2[0][0];
+        stats[2 * st] = (s1 != 0.0) ? n * s2 / (s1 * s1) : 0.0;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats[2 * st + 1] = ms;
+      }
+    }
+    l->tables_valid = false;
+    ensure_tables(c, l);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_tables(ftr_ctx* c, ftr_learner* l, double* lw, double* lZ, double* lratio, double* lia,
+                       double* lrad, double* ldiff) {
+  if (!c || !l) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    ensure_tables(c, l);
+    const size_t NT = l->NT, E = l->E;
+    if (lw) HIPCHK(hipMemcpyDtoHAsync(lw, l->d_lw, NT * E * 8, c->stream));
+    if (lratio) HIPCHK(hipMemcpyDtoHAsync(lratio, l->d_lratio, NT * E * 8, c->stream));
+    if (lZ) HIPCHK(hipMemcpyDtoHAsync(lZ, l->d_lZ, NT * 8, c->stream));
+    if (lia) HIPCHK(hipMemcpyDtoHAsync(lia, l->d_lia, NT * 8, c->stream));
+    if (lrad) HIPCHK(hipMemcpyDtoHAsync(lrad, l->d_lrad, NT * 8, c->stream));
+    if (ldiff) HIPCHK(hipMemcpyDtoHAsync(ldiff, l->d_ldiff, NT * 8, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_integrate(ftr_ctx* c, ftr_learner* l, uint64_t N, uint64_t seed, const ftr_opts* opts,
+                          ftr_moments* out) {
+  if (!c || !l || !out) return FTR_ERR_INVALID;
+  ftr_problem* p = l->p;
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(c->device));
+    const int K = p->prep.K, NQ = 2 + 4 * K;
+    memset(out, 0, sizeof *out);
+    out->K = K;
+    if (opts && opts->block && opts->block != l->var.block)
+      throw Error(FTR_ERR_INVALID, "block size differs from the compiled variant");
+    ensure_tables(c, l);
+    if (N > 0) {
+      Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, l->var.block, NQ, l->f_lev);
+      ftr::LearnLaunch L = learn_launch(l, G, N);
+      void* args[] = {&L};
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      HIPCHK(hipModuleLaunchKernel(l->f_lev, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
+                                   c->stream, args, nullptr));
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      reduce_partials(c, G, K, p->prep.path, out);
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      out->kernel_ms = ms;
+    } else if (p->prep.path != 2) {
+      for (int k = 0; k < K; k++) out->n[k][1] = 2;
+    }
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_sample(ftr_ctx* c, ftr_learner* l, uint64_t first, uint64_t n, uint64_t seed, double* X_dev,
+                       double* w_dev, uint64_t* path_dev) {
+  if (!c || !l || (n && (!X_dev || !w_dev || !path_dev))) return FTR_ERR_INVALID;
+  ftr_problem* p = l->p;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    ensure_tables(c, l);
+    if (n == 0) return FTR_OK;
+    Geometry G = setup_stream(p, first, n, seed, 0, l->var.block, 2 + 4 * p->prep.K, l->f_lsm);
+    ftr::LearnLaunch L = learn_launch(l, G, n);
+    void* args[] = {&L, &X_dev, &w_dev, &path_dev};
+    HIPCHK(hipModuleLaunchKernel(l->f_lsm, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
+                                 c->stream, args, nullptr));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_precompile_learned(const ftr_input* in, char* err, int errlen) {
+  try {
+    ftrh::Prep pr;
+    prep_from_input(pr, in);
+    if (pr.g.E > MAX_E) throw Error(FTR_ERR_LIMIT, "the learned proposal supports at most 16 edges");
+    code_object(ftr_embedded_mlp_source, nullptr);
+    for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024})
+      code_object(full_source(pr, choose_variant(pr, lim), true), nullptr);
+  } catch (std::exception& e) {
+    if (err && errlen > 0) {
+      strncpy(err, e.what(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fe ? fe->code : FTR_ERR_INVALID;
+  }
+  return FTR_OK;
+}
+
+}  // extern "C"

@@ -28,62 +28,9 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/feyntrop_amd.h"
-#include "../kernels/ftr_abi.hpp"
-#include "ftr_host.hpp"
+#include "ftr_runtime.hpp"
 
-extern const char* ftr_embedded_device_source;
-
-using ftrh::Error;
-
-struct Variant {
-  int mode = 0, tmode = 0, block = 256, K = 1;
-};
-
-struct Module {
-  hipModule_t mod = nullptr;
-  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr, f_bat = nullptr;
-};
-
-struct ftr_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipDeviceProp_t prop;
-  std::string err;
-  std::map<std::string, Module> modules;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  /* growable device scratch: block states, lane polys, partials */
-  void* d_scratch = nullptr;
-  size_t scratch_bytes = 0;
-  std::vector<double> h_partials;
-  /* batched scans */
-  void* d_batch = nullptr;
-  size_t batch_bytes = 0;
-  double* h_pinned = nullptr; /* pinned copy-back buffer of the batch graph */
-  size_t pinned_bytes = 0;
-};
-
-struct ftr_problem {
-  ftrh::Prep prep;
-  ftr_ctx* ctx = nullptr;
-  int device = 0;
-  Variant var;
-  size_t lds = 0;
-  Module m;
-  ftr::DevParams hprm;
-  ftr::DevParams* d_prm = nullptr;
-  double *d_w = nullptr, *d_jr = nullptr, *d_ir = nullptr;
-  uint8_t* d_mml = nullptr;
-};
-
-/* ------------------------------------------------------------ helpers -- */
-#define HIPCHK(x)                                                                            \
-  do {                                                                                       \
-    hipError_t e_ = (x);                                                                     \
-    if (e_ != hipSuccess) throw Error(FTR_ERR_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-static thread_local std::string g_host_err; /* errors of ctx-less calls */
+thread_local std::string g_host_err; /* errors of ctx-less calls */
 static std::string g_cache_dir;
 static std::mutex g_cache_mu;
 
@@ -115,7 +62,7 @@ static uint64_t fnv1a(const std::string& s) {
 }
 
 /* Kernel variant for a problem: path -> MODE, table size -> placement. */
-static Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
+Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
   Variant v;
   v.K = pr.K;
   v.mode = pr.path;
@@ -162,7 +109,7 @@ static size_t variant_lds(const Variant& v, int E) {
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
 }
 
-static std::string spec_source(const ftrh::Prep& pr, const Variant& v) {
+static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool learned = false) {
   const ftrh::Graph& g = pr.g;
   /* elimination order: internal vertices, then those with nonzero P rows */
   std::vector<int> ord, pos(g.V);
@@ -185,6 +132,8 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v) {
   arr(s, "ord", ord);
   arr(s, "pos", pos);
   s << "};\nFTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
+  if (learned)
+    s << "FTR_INSTANTIATE_LEARNED(ftrl, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
   return s.str();
 }
 
@@ -204,7 +153,7 @@ static const std::vector<std::string>& rtc_options() {
 }
 
 /* Compile (or fetch from the cache) the code object for a source. */
-static std::string code_object(const std::string& src, std::string* path_out) {
+std::string code_object(const std::string& src, std::string* path_out) {
   std::string key = src;
   for (auto& o : rtc_options()) key += "\n" + o;
   int rtcver = 0, rtcmin = 0;
@@ -279,8 +228,8 @@ static bool load_tuned(const ftrh::Prep& pr, size_t lds_limit, Variant* out) {
   return false;
 }
 
-static std::string full_source(const ftrh::Prep& pr, const Variant& v) {
-  std::string src = std::string(ftr_embedded_device_source) + spec_source(pr, v);
+std::string full_source(const ftrh::Prep& pr, const Variant& v, bool learned) {
+  std::string src = std::string(ftr_embedded_device_source) + spec_source(pr, v, learned);
   if (const char* d = getenv("FEYNTROP_AMD_DUMP_SRC")) {
     char name[64];
     snprintf(name, sizeof name, "/ftr_%016llx.hip", (unsigned long long)fnv1a(src));
@@ -312,7 +261,7 @@ static void fill_params(ftr_problem* p) {
     for (int j = 0; j < pr.g.V; j++) d.P[i * FTR_MAX_V + j] = pr.P[(size_t)i * pr.g.V + j];
 }
 
-static void prep_from_input(ftrh::Prep& pr, const ftr_input* in) {
+void prep_from_input(ftrh::Prep& pr, const ftr_input* in) {
   if (!in || in->E < 1) throw Error(FTR_ERR_INVALID, "input graph must have at least one edge");
   ftrh::build_graph(pr.g, in->E, in->edges, in->nu);
   if (in->V != pr.g.V) throw Error(FTR_ERR_INVALID, "input scalarproducts must be a (V x V)-matrix");
@@ -328,7 +277,7 @@ static void prep_from_input(ftrh::Prep& pr, const ftr_input* in) {
   if (pr.g.E > FTR_MAX_E) throw Error(FTR_ERR_LIMIT, "more than 30 edges are not supported");
 }
 
-static void ensure_scratch(ftr_ctx* c, size_t bytes) {
+void ensure_scratch(ftr_ctx* c, size_t bytes) {
   if (bytes <= c->scratch_bytes) return;
   if (c->d_scratch) HIPCHK(hipFree(c->d_scratch));
   c->d_scratch = nullptr;
@@ -337,12 +286,21 @@ static void ensure_scratch(ftr_ctx* c, size_t bytes) {
   c->scratch_bytes = nb;
 }
 
+hipModule_t load_module_raw(ftr_ctx* c, const std::string& src) {
+  auto it = c->raw_modules.find(src);
+  if (it != c->raw_modules.end()) return it->second;
+  const std::string co = code_object(src, nullptr);
+  hipModule_t mod = nullptr;
+  HIPCHK(hipModuleLoadData(&mod, co.data()));
+  c->raw_modules[src] = mod;
+  return mod;
+}
+
 static Module load_module(ftr_ctx* c, const std::string& src) {
   auto it = c->modules.find(src);
   if (it != c->modules.end()) return it->second;
-  const std::string co = code_object(src, nullptr);
   Module m;
-  HIPCHK(hipModuleLoadData(&m.mod, co.data()));
+  m.mod = load_module_raw(c, src);
   HIPCHK(hipModuleGetFunction(&m.f_int, m.mod, "ftrk_int"));
   HIPCHK(hipModuleGetFunction(&m.f_rep, m.mod, "ftrk_rep"));
   HIPCHK(hipModuleGetFunction(&m.f_smp, m.mod, "ftrk_smp"));
@@ -351,25 +309,15 @@ static Module load_module(ftr_ctx* c, const std::string& src) {
   return m;
 }
 
-/* Geometry + stream positioning for a run of n samples starting at sample
- * `first` of the stream seeded with `seed`.  Writes block states and lane
- * polynomials to the context scratch; returns (nblocks, S). */
-struct Geometry {
-  uint64_t nblocks = 0, S = 0, lanes = 0;
-  uint64_t* d_states = nullptr;
-  uint64_t* d_polys = nullptr;
-  double* d_partials = nullptr;
-};
-
-static Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req,
-                             int block, int nq) {
+Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
+                      int nq, hipFunction_t occ_fn) {
   ftr_ctx* c = p->ctx;
   const int E = p->prep.g.E;
   Geometry G;
   uint64_t lanes_target = lanes_req;
   if (!lanes_target) {
     int nbpc = 0;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, p->m.f_int, block, p->lds) != hipSuccess || nbpc < 1)
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, occ_fn ? occ_fn : p->m.f_int, block, p->lds) != hipSuccess || nbpc < 1)
       nbpc = 1;
     lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * block;
   }
@@ -414,7 +362,7 @@ static Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_
   return G;
 }
 
-static int fail(ftr_ctx* c, int code, const std::string& msg) {
+int fail(ftr_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
 }
@@ -448,7 +396,7 @@ int ftr_create(int device, ftr_ctx** out) {
 void ftr_destroy(ftr_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  for (auto& kv : c->modules) (void)hipModuleUnload(kv.second.mod);
+  for (auto& kv : c->raw_modules) (void)hipModuleUnload(kv.second);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
   if (c->d_batch) (void)hipFree(c->d_batch);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -597,7 +545,7 @@ int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const f
     if (opts && opts->block && opts->block != p->var.block)
       throw Error(FTR_ERR_INVALID, "block size differs from the compiled variant");
     if (N > 0) {
-      Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ);
+      Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ, nullptr);
       ftr::Launch a;
       a.prm = p->d_prm;
       a.t_w = p->d_w;
@@ -669,7 +617,7 @@ int ftr_sample(ftr_ctx* c, ftr_problem* p, uint64_t first, uint64_t n, uint64_t 
   try {
     HIPCHK(hipSetDevice(c->device));
     if (n == 0) return FTR_OK;
-    Geometry G = setup_stream(p, first, n, seed, 0, p->var.block, 2 + 4 * p->prep.K);
+    Geometry G = setup_stream(p, first, n, seed, 0, p->var.block, 2 + 4 * p->prep.K, nullptr);
     ftr::Launch a;
     memset(&a, 0, sizeof a);
     a.prm = p->d_prm;

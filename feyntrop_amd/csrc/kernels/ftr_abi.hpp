@@ -58,4 +58,27 @@ struct BatchItem {
   uint32_t nblocks;
 };
 
+/* Learned proposal tables (2^E states g; E <= 16):
+ *   lw[g*E+e]     cut weight of e from g: w(g\e) exp(Delta_e(g))
+ *   lZ[g]         sum of lw[g*E+e] over e in g, ascending e
+ *   lratio[g*E+e] p_trop(e|g) / q(e|g) = lZ[g] w(g\e) / (J(g) lw[g*E+e])
+ *   lia[g]        1 / a(g), a(g) = r(g) exp(sigma(g)): radial exponent after
+ *                 arriving at g (plain sampling: a = r)
+ *   lrad[g]       log(r/a) = -sigma(g);   ldiff[g]  r(g) - a(g)
+ * Training accumulators (doubles): gC[g*E+e] += c for every visit of g that
+ * cut e; gS[g] += c (1 + a(g) log Y) for the radius drawn after g;
+ * gsum[0] += c, gsum[1] += 1, with c = weight * |R_0|. */
+struct LearnLaunch {
+  Launch a;
+  const double* lw;
+  const double* lZ;
+  const double* lratio;
+  const double* lia;
+  const double* lrad;
+  const double* ldiff;
+  double* gC;
+  double* gS;
+  double* gsum;
+};
+
 }  // namespace ftr

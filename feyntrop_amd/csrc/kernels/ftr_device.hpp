@@ -750,6 +750,208 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
   }
 }
 
+/* Learned proposal: the cut and the radius distributions come from the
+ * learned tables; the exact FP64 ratio p_trop/q of the drawn path multiplies
+ * the integrand, so E_q[R p/q] is the same integral.  Same 2E-1 draws per
+ * sample as the plain sampler.  GRAD: also accumulate the forward-KL
+ * statistics of the drawn paths. */
+template <class G, int K, int MODE, int TMODE, bool GRAD>
+__device__ inline void learn_body(const LearnLaunch& L) {
+  extern __shared__ __align__(16) char lds[];
+  constexpr int E = G::E;
+  constexpr int NQ = 2 + 4 * K;
+  const Launch& a = L.a;
+  const DevParams& p = *a.prm;
+  TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
+  double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
+  Xoshiro rng = lane_stream(a, blockIdx.x);
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i0 = lane * a.S;
+  const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+  double gc0[GRAD ? E : 1], gs0[GRAD ? E : 1], gsum0 = 0.0, gsum1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < (GRAD ? E : 1); k++) gc0[k] = gs0[k] = 0.0;
+  for (uint64_t i = i0; i < i1; i++) {
+    double X[E];
+    uint32_t g = (1u << E) - 1u;
+    const uint32_t b0 = tv.mmL(g);
+    bool MM = b0 & 1;
+    int oL = (int)(b0 >> 1);
+    double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
+    uint64_t path = 0;
+    double lYs[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      X[e] = 0.0;
+      lYs[e] = 0.0;
+    }
+    for (int j = 0; j < E; j++) {
+      const double R = rng.uniform() * L.lZ[g];
+      const double* row = L.lw + (size_t)g * E;
+      double T = 0.0;
+      int e = 0;
+      uint32_t m = g;
+      while (m) {
+        const int q = __builtin_ctz(m);
+        m &= m - 1;
+        T += row[q];
+        e = q;
+        if (T > R) break;
+      }
+      ratio *= L.lratio[(size_t)g * E + e];
+      const uint32_t c = g ^ (1u << e);
+      const uint32_t b = tv.mmL(c);
+      const bool mm = b & 1;
+      const int Lc = (int)(b >> 1);
+      if (MM && !mm) lGv = lX;
+      MM = mm;
+#pragma unroll
+      for (int k = 0; k < E; k++)
+        if (k == e) X[k] = XX;
+      if (Lc < oL) {
+        lF += lX;
+        oL = Lc;
+      }
+      if (GRAD) path |= (uint64_t)e << (4 * j);
+      if (j < E - 1) {
+        const double U = rng.uniform();
+        const double lY = L.lia[c] * log(1. - U);
+        lX += lY;
+        XX = exp(lX);
+        lrad += L.lrad[c] + L.ldiff[c] * lY;
+        if (GRAD) {
+#pragma unroll
+          for (int k = 0; k < E; k++)
+            if (k == j) lYs[k] = lY;
+        }
+      }
+      g = c;
+    }
+    const double wgt = ratio * exp(lrad);
+    cplx out[K];
+    if (!integrand<G, K, MODE>(p, X, log_scale(p, lF, lGv), out)) {
+      acc[1] += 1.0;
+      continue;
+    }
+    acc[0] += 1.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double re = out[k].re * wgt, im = out[k].im * wgt;
+      acc[2 + 4 * k] += re;
+      acc[3 + 4 * k] += re * re;
+      acc[4 + 4 * k] += im;
+      acc[5 + 4 * k] += im * im;
+    }
+    if (GRAD) {
+      /* the first cut (from the full graph) and the radius after it hit E
+       * addresses shared by every sample: accumulated per lane and added
+       * once per wave; deeper states spread over many addresses */
+      const double cs = wgt * sqrt(out[0].re * out[0].re + out[0].im * out[0].im);
+      const int e0 = (int)(path & 15u);
+      const uint32_t c0 = ((1u << E) - 1u) ^ (1u << e0);
+      const double s0v = (E > 1) ? cs * (1.0 + lYs[0] / L.lia[c0]) : 0.0;
+#pragma unroll
+      for (int k = 0; k < E; k++)
+        if (k == e0) {
+          gc0[k] += cs;
+          gs0[k] += s0v;
+        }
+      gsum0 += cs;
+      gsum1 += 1.0;
+      uint32_t gg = c0;
+#pragma unroll
+      for (int j = 1; j < E; j++) {
+        const int e = (int)((path >> (4 * j)) & 15u);
+        unsafeAtomicAdd(&L.gC[(size_t)gg * E + e], cs);
+        const uint32_t c = gg ^ (1u << e);
+        if (j < E - 1) unsafeAtomicAdd(&L.gS[c], cs * (1.0 + lYs[j] / L.lia[c]));
+        gg = c;
+      }
+    }
+  }
+  if (GRAD) {
+    const int lid0 = threadIdx.x & 63;
+    const uint32_t full = (1u << E) - 1u;
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+      const double vc = wave_sum(gc0[k]);
+      const double vs = wave_sum(gs0[k]);
+      if (lid0 == 0 && vc != 0.0) {
+        unsafeAtomicAdd(&L.gC[(size_t)full * E + k], vc);
+        if (E > 1) unsafeAtomicAdd(&L.gS[full ^ (1u << k)], vs);
+      }
+    }
+    const double t0 = wave_sum(gsum0), t1 = wave_sum(gsum1);
+    if (lid0 == 0) {
+      unsafeAtomicAdd(&L.gsum[0], t0);
+      unsafeAtomicAdd(&L.gsum[1], t1);
+    }
+  }
+  const int wid = threadIdx.x >> 6, lid = threadIdx.x & 63, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const double v = wave_sum(acc[q]);
+    if (lid == 0) red[wid * NQ + q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    double sm = 0.0;
+    for (int w = 0; w < nw; w++) sm += red[w * NQ + threadIdx.x];
+    a.partials[(size_t)blockIdx.x * NQ + threadIdx.x] = sm;
+  }
+}
+
+/* Learned sampler only (parity): X, the ratio weight and the cut order
+ * (4 bits per step) of samples [first, first+n) of the stream. */
+template <class G, int TMODE>
+__device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, double* wout, uint64_t* pout) {
+  extern __shared__ __align__(16) char lds[];
+  constexpr int E = G::E;
+  const Launch& a = L.a;
+  TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
+  (void)tv;
+  Xoshiro rng = lane_stream(a, blockIdx.x);
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i0 = lane * a.S;
+  const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
+  for (uint64_t i = i0; i < i1; i++) {
+    uint32_t g = (1u << E) - 1u;
+    double lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
+    uint64_t path = 0;
+    for (int j = 0; j < E; j++) {
+      const double R = rng.uniform() * L.lZ[g];
+      const double* row = L.lw + (size_t)g * E;
+      double T = 0.0;
+      int e = 0;
+      uint32_t m = g;
+      while (m) {
+        const int q = __builtin_ctz(m);
+        m &= m - 1;
+        T += row[q];
+        e = q;
+        if (T > R) break;
+      }
+      ratio *= L.lratio[(size_t)g * E + e];
+      const uint32_t c = g ^ (1u << e);
+      Xout[i * E + e] = XX;
+      path |= (uint64_t)e << (4 * j);
+      if (j < E - 1) {
+        const double U = rng.uniform();
+        const double lY = L.lia[c] * log(1. - U);
+        lX += lY;
+        XX = exp(lX);
+        lrad += L.lrad[c] + L.ldiff[c] * lY;
+      }
+      g = c;
+    }
+    wout[i] = ratio * exp(lrad);
+    pout[i] = path;
+  }
+}
+
 /* Batched scan: every block belongs to one problem of the item list (found
  * by binary search on block0; uniform per block). */
 template <class G, int K, int MODE, int TMODE>
@@ -765,6 +967,19 @@ __device__ inline void batch_body(const BatchItem* items, int nitems) {
 }
 
 }  // namespace ftr
+
+/* Learned-proposal kernels (compiled only for problems that use them). */
+#define FTR_INSTANTIATE_LEARNED(NAME, G, K, MODE, TMODE, BLOCK)                                   \
+  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_lev(ftr::LearnLaunch L) {              \
+    ftr::learn_body<G, K, MODE, TMODE, false>(L);                                                 \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_lgr(ftr::LearnLaunch L) {              \
+    ftr::learn_body<G, K, MODE, TMODE, true>(L);                                                  \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_lsm(ftr::LearnLaunch L, double* X,     \
+                                                                   double* w, uint64_t* path) {   \
+    ftr::learn_sample_body<G, TMODE>(L, X, w, path);                                              \
+  }
 
 /* Instantiation macro used by the generated kernel units:
  *   FTR_INSTANTIATE(name, Graph, K, MODE, TMODE, BLOCK)

@@ -1,0 +1,213 @@
+/*
+ * ftr_mlp.hpp -- the learned proposal's network on gfx950 MFMA (no reference
+ * counterpart).  A 4-layer MLP (input 32, three hidden layers of 128 SiLU
+ * units, output 32) in BF16 with FP32 accumulation:
+ *   Z1 = X W1^T + b1, H1 = silu(Z1); Z2, H2; Z3, H3; Y = H3 W4^T + b4.
+ * Every dense product is v_mfma_f32_32x32x8_bf16 (one wave per 32x32 output
+ * tile); forward and backward use the same kernel with strided operands.
+ *
+ * MFMA lane layout used (checked on the device): for lane l,
+ *   A (32x8):  A[l%32][4(l/32)+t], t = 0..3
+ *   B (8x32):  B[4(l/32)+t][l%32]
+ *   C (32x32): acc v -> C[8(v/4) + 4(l/32) + v%4][l%32]
+ */
+#pragma once
+
+#ifdef __HIPCC_RTC__
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+#else
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#endif
+
+namespace ftrm {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+/* round-to-nearest-even float -> bf16 bits */
+__device__ inline unsigned short f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);  /* NaN */
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ inline float dsilu(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+/* Operand access: element (i, k) of A lives at A[i*si + k*sk]; AT is the
+ * storage type (unsigned short = bf16 bits, or float converted on load). */
+template <class AT>
+__device__ inline short ld_bf(const AT* p, size_t off);
+template <>
+__device__ inline short ld_bf<unsigned short>(const unsigned short* p, size_t off) {
+  return (short)p[off];
+}
+template <>
+__device__ inline short ld_bf<float>(const float* p, size_t off) {
+  return (short)f2bf(p[off]);
+}
+
+/* C[M x N] = act(A[M x K] B[K x N] + bias)  (M, N multiples of 32, K of 8).
+ * Writes the FP32 result to C (if non-null) and its bf16 rounding after the
+ * activation to Cb (if non-null).  act: 0 identity, 1 SiLU.  One wave per
+ * output tile; grid = (N/32, M/32, splits) blocks of 64 threads.  With
+ * splits > 1 block z covers K rows [z kc, (z+1) kc) and adds its partial
+ * product to C atomically (C zeroed beforehand; bias, act and Cb unused). */
+template <class AT, class BT>
+__device__ inline void gemm_tile(const AT* A, size_t sai, size_t sak, const BT* B, size_t sbk, size_t sbj,
+                                 const float* bias, float* C, size_t ldc, unsigned short* Cb, size_t ldcb,
+                                 int K, int act, int kc) {
+  const int l = threadIdx.x;
+  const int ti = blockIdx.y, tj = blockIdx.x;
+  const int ar = ti * 32 + (l & 31), bc = tj * 32 + (l & 31), kq = 4 * (l >> 5);
+  const bool split = gridDim.z > 1;
+  const int kbeg = split ? (int)blockIdx.z * kc : 0;
+  const int kend = split ? (kbeg + kc < K ? kbeg + kc : K) : K;
+  f32x16 acc;
+#pragma unroll
+  for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+  for (int k0 = kbeg; k0 < kend; k0 += 8) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      a[t] = ld_bf<AT>(A, (size_t)ar * sai + (size_t)(k0 + kq + t) * sak);
+      b[t] = ld_bf<BT>(B, (size_t)(k0 + kq + t) * sbk + (size_t)bc * sbj);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int i = ti * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
+    const int j = bc;
+    if (split) {
+      unsafeAtomicAdd(&C[(size_t)i * ldc + j], acc[v]);
+      continue;
+    }
+    const float z = acc[v] + (bias ? bias[j] : 0.0f);
+    if (C) C[(size_t)i * ldc + j] = z;
+    if (Cb) Cb[(size_t)i * ldcb + j] = f2bf(act == 1 ? silu(z) : z);
+  }
+}
+
+}  // namespace ftrm
+
+extern "C" __global__ void __launch_bounds__(64)
+    mlp_gemm_bb(const unsigned short* A, uint64_t sai, uint64_t sak, const unsigned short* B, uint64_t sbk,
+                uint64_t sbj, const float* bias, float* C, uint64_t ldc, unsigned short* Cb, uint64_t ldcb, int K,
+                int act, int kc) {
+  ftrm::gemm_tile<unsigned short, unsigned short>(A, sai, sak, B, sbk, sbj, bias, C, ldc, Cb, ldcb, K, act, kc);
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+    mlp_gemm_fb(const float* A, uint64_t sai, uint64_t sak, const unsigned short* B, uint64_t sbk, uint64_t sbj,
+                const float* bias, float* C, uint64_t ldc, unsigned short* Cb, uint64_t ldcb, int K, int act,
+                int kc) {
+  ftrm::gemm_tile<float, unsigned short>(A, sai, sak, B, sbk, sbj, bias, C, ldc, Cb, ldcb, K, act, kc);
+}
+
+/* dZ = dH * silu'(Z) elementwise (n values). */
+extern "C" __global__ void mlp_silu_bwd(const float* dH, const float* Z, float* dZ, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dZ[i] = dH[i] * ftrm::dsilu(Z[i]);
+}
+
+/* db[j] += sum_i dZ[i][j]  (rows x cols, row-major; db zeroed beforehand):
+ * one thread per column and chunk of `chunk` rows (grid.y = row chunks). */
+extern "C" __global__ void mlp_colsum(const float* dZ, float* db, int rows, int cols, int chunk) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  const int i0 = blockIdx.y * chunk;
+  const int i1 = i0 + chunk < rows ? i0 + chunk : rows;
+  float s = 0.0f;
+  for (int i = i0; i < i1; i++) s += dZ[(size_t)i * cols + j];
+  unsafeAtomicAdd(&db[j], s);
+}
+
+/* Adam on FP32 master parameters, refreshing their bf16 copies. */
+extern "C" __global__ void mlp_adam(float* w, unsigned short* wb, const float* g, float* m, float* v, uint64_t n,
+                                    float lr, float b1, float b2, float eps, float c1, float c2) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = ftrm::f2bf(wi);
+}
+
+/* fill (instead of hipMemset, so the sequence can live in a graph) */
+extern "C" __global__ void mlp_fill(double* p, uint64_t n, double val) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = val;
+}
+
+extern "C" __global__ void mlp_fill_f(float* p, uint64_t n, float val) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = val;
+}
+
+/* float -> bf16 */
+extern "C" __global__ void mlp_to_bf16(const float* src, unsigned short* dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = ftrm::f2bf(src[i]);
+}
+
+/* Learned tables from the network outputs Y (rows = states g, 32 columns:
+ * Delta_e in columns 0..E-1, sigma in column E), squashed to
+ * Delta = dmax tanh(y/dmax), sigma = smax tanh(y/smax). */
+extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double* w, const double* jr,
+                                            const double* r, double* lw, double* lZ, double* lratio, double* lia,
+                                            double* lrad, double* ldiff, double dmax, double smax) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (1u << E)) return;
+  const float* y = Y + (size_t)g * 32;
+  double Z = 0.0;
+  for (int e = 0; e < E; e++) {
+    double v = 0.0;
+    if ((g >> e) & 1u) v = w[g ^ (1u << e)] * exp(dmax * tanh((double)y[e] / dmax));
+    lw[(size_t)g * E + e] = v;
+    Z += v;
+  }
+  lZ[g] = Z;
+  for (int e = 0; e < E; e++) {
+    const double v = lw[(size_t)g * E + e];
+    lratio[(size_t)g * E + e] = ((g >> e) & 1u) ? (Z * w[g ^ (1u << e)]) / (jr[g] * v) : 0.0;
+  }
+  const double sg = smax * tanh((double)y[E] / smax);
+  const double aa = r[g] * exp(sg);
+  lia[g] = 1.0 / aa;
+  lrad[g] = -sg;
+  ldiff[g] = r[g] - aa;
+}
+
+/* dL/dY of the forward KL  L = -sum_paths c log q(path) / sum c  from the
+ * accumulated statistics (see ftr_abi.hpp LearnLaunch). */
+extern "C" __global__ void mlp_table_grad(const float* Y, int E, const double* lw, const double* lZ, const double* gC,
+                                          const double* gS, const double* gsum, float* dY, double dmax, double smax) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (1u << E)) return;
+  const float* y = Y + (size_t)g * 32;
+  float* d = dY + (size_t)g * 32;
+  const double S = gsum[0] > 0.0 ? gsum[0] : 1.0;
+  double visits = 0.0;
+  for (int e = 0; e < E; e++) visits += gC[(size_t)g * E + e];
+  for (int c = 0; c < 32; c++) d[c] = 0.0f;
+  for (int e = 0; e < E; e++) {
+    if (!((g >> e) & 1u)) continue;
+    const double q = lw[(size_t)g * E + e] / lZ[g];
+    const double t = tanh((double)y[e] / dmax);
+    d[e] = (float)(-(gC[(size_t)g * E + e] - q * visits) / S * (1.0 - t * t));
+  }
+  const double t = tanh((double)y[E] / smax);
+  d[E] = (float)(-gS[g] / S * (1.0 - t * t));
+}

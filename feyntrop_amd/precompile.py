@@ -9,7 +9,7 @@ import concurrent.futures as cf
 import os
 import sys
 
-from feyntrop_amd import host_path, precompile, workloads
+from feyntrop_amd import host_path, precompile, precompile_learned, workloads
 
 
 def problems():
@@ -20,16 +20,21 @@ def problems():
 
 
 def _one(item):
-    name, fn = item
+    name, fn, learned = item
     try:
-        precompile(fn() if callable(fn) else fn)
+        prob = fn() if callable(fn) else fn
+        precompile(prob)
+        if learned and len(prob["graph"]) <= 16:
+            precompile_learned(prob)
         return name, None
     except Exception as e:  # report, keep going
         return name, str(e)
 
 
 def main():
-    items = list(problems().items())
+    named = set(workloads.all_problems())
+    # learned-proposal kernels for the named workloads only
+    items = [(k, v, k in named) for k, v in problems().items()]
     workers = min(8, os.cpu_count() or 1)
     bad = 0
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:

@@ -149,6 +149,49 @@ int ftr_set_cache_dir(const char* dir);
 /* Compile (hipRTC) the kernels for a problem into the cache without a GPU. */
 int ftr_precompile(const ftr_input* in, char* err, int errlen);
 
+/* ---- Learned proposal (no reference counterpart: an addition on top of
+ * the tropical sampler of feynman_integral.hpp:31-77) ----
+ * A small MLP (input 32, 3 x 128 SiLU, output 32; BF16 weights on the MFMA
+ * units, FP32 masters, Adam) maps every subgraph state g of the J-table to
+ * a tilt Delta_e(g) of each cut weight and a radial exponent shift
+ * sigma(g).  Sampling from the tilted proposal q and weighting every sample
+ * by the exact FP64 ratio p_trop(path)/q(path) keeps the estimator unbiased
+ * for any network; training minimises the forward KL divergence between
+ * |integrand| x p_trop and q.  Needs E <= 16.  An untrained learner
+ * (zero output layer) reproduces the plain sampler sample for sample. */
+typedef struct ftr_learner ftr_learner;
+#define FTR_LEARN_HIDDEN 128
+#define FTR_LEARN_NPARAMS (128 * 32 + 128 + 2 * (128 * 128 + 128) + 32 * 128 + 32)
+
+int ftr_learner_create(ftr_ctx* ctx, ftr_problem* p, uint64_t init_seed, ftr_learner** out);
+void ftr_learner_free(ftr_learner* l);
+/* FP32 master parameters (FTR_LEARN_NPARAMS floats: W1[128x32] b1 W2[128x128]
+ * b2 W3[128x128] b3 W4[32x128] b4); get and/or set (either may be NULL). */
+int ftr_learner_params(ftr_learner* l, float* get, const float* set);
+/* Network output for every state: Y (max(32, 2^E) x 32 floats, host). */
+int ftr_learner_forward(ftr_ctx* ctx, ftr_learner* l, float* Y);
+/* Parameter gradient of sum(dY * Y) for a given dY (same shape as Y). */
+int ftr_learner_backward(ftr_ctx* ctx, ftr_learner* l, const float* dY, float* grad);
+/* `steps` Adam steps, each on `samples` proposal samples (stream seed
+ * seed + step).  stats (optional, steps x 2): per step the relative
+ * second moment N sum(w f)^2 / (sum w f)^2 of the epsilon^0 real part and
+ * the kernel milliseconds. */
+int ftr_learner_train(ftr_ctx* ctx, ftr_learner* l, int steps, uint64_t samples, uint64_t seed, double lr,
+                      double* stats);
+/* Current tables, host arrays: lw, lratio (2^E x E); lZ, lia, lrad, ldiff (2^E). */
+int ftr_learner_tables(ftr_ctx* ctx, ftr_learner* l, double* lw, double* lZ, double* lratio, double* lia,
+                       double* lrad, double* ldiff);
+/* Integrate with the learned proposal: moments of R * p_trop/q. */
+int ftr_learner_integrate(ftr_ctx* ctx, ftr_learner* l, uint64_t N, uint64_t seed, const ftr_opts* opts,
+                          ftr_moments* out);
+/* Proposal samples [first, first+n): X (n*E), weight p/q (n), cut order
+ * (n, 4 bits per step, first cut lowest) into device buffers. */
+int ftr_learner_sample(ftr_ctx* ctx, ftr_learner* l, uint64_t first, uint64_t n, uint64_t seed, double* X_dev,
+                       double* w_dev, uint64_t* path_dev);
+/* Compile the learned-proposal kernels of a problem (and the network's)
+ * into the cache without a GPU. */
+int ftr_precompile_learned(const ftr_input* in, char* err, int errlen);
+
 #ifdef __cplusplus
 }
 #endif

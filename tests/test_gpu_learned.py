@@ -1,0 +1,162 @@
+"""GPU tests of the learned proposal (ftr_learner_* through the C ABI)
+against the numpy restatement in learn_ref.py, which the CPU tests pin
+(finite differences, zero tilt = the oracle's tropical sampler, weights
+computed two ways, E_q[p/q] = 1).
+
+Bars (written in each test):
+  network        forward Y and the parameter gradient against numpy with the
+                 kernels' bf16 roundings emulated: max error <= 2e-3 of the
+                 block's largest entry (FP32 accumulation order, bf16 ties)
+  tables         FP64 tables against numpy on the GPU's Y: 1e-13 relative
+  sampler        cut order identical, X and the weight p/q within 1e-12
+                 relative of the numpy replay on the same uniforms; the
+                 weight within 1e-9 of p_trop/q from first principles
+  untrained      zero output layer: the estimate of the plain sampler, same
+                 samples (moments within 1e-12 relative)
+  statistics     trained estimates within 3 combined standard errors of the
+                 plain ones; K4 wheel = 6 zeta(3); training lowers the
+                 relative second moment
+"""
+
+import numpy as np
+import pytest
+
+import feyntrop_amd
+import learn_ref as LR
+from feyntrop_amd import workloads as W
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(engine, name):
+    prob = W.all_problems()[name]()
+    p = engine.prepare(prob)
+    return prob, p
+
+
+def _rand_par(rng, out_scale=0.05):
+    par = rng.normal(0.0, 0.15, LR.NPARAMS)
+    d = LR.unpack(par)
+    d["W4"] = rng.normal(0, out_scale, d["W4"].shape)
+    d["b4"] = rng.normal(0, out_scale, d["b4"].shape)
+    return LR.pack(d).astype(np.float32)
+
+
+def _blockwise_close(a, b, frac):
+    o = 0
+    for name, s in LR.SHAPES:
+        n = int(np.prod(s))
+        x, y = a[o:o + n], b[o:o + n]
+        scale = max(np.abs(y).max(), 1e-30)
+        assert np.abs(x - y).max() <= frac * scale, (name, np.abs(x - y).max(), scale)
+        o += n
+
+
+@pytest.mark.parametrize("name", ["double_box", "triple_ladder", "1L_2pt"])
+def test_network_forward_and_backward(engine, name):
+    prob, p = _setup(engine, name)
+    lr = feyntrop_amd.Learner(engine, p, init_seed=7)
+    rng = np.random.default_rng(2)
+    par = _rand_par(rng)
+    lr.set_params(par)
+    np.testing.assert_array_equal(lr.params(), par)
+    Jr, r, _, _ = p.jtable()
+    X = LR.features(p.E, r)
+    Y = lr.forward().astype(np.float64)
+    Yr, cache = LR.forward(par.astype(np.float64), X, emulate=True)
+    n = 1 << p.E
+    scale = np.abs(Yr[:n]).max()
+    assert np.abs(Y[:n] - Yr[:n]).max() <= 2e-3 * scale
+    dY = rng.normal(0, 1, Y.shape)
+    dY[n:] = 0.0
+    dY = dY.astype(np.float32)
+    g = lr.backward(dY).astype(np.float64)
+    gr = LR.backward(par.astype(np.float64), cache, dY.astype(np.float64), emulate=True)
+    _blockwise_close(g, gr, 2e-3)
+
+
+@pytest.mark.parametrize("name", ["double_box", "2L_3pt_tutorial"])
+def test_tables_and_sampler_replay(engine, oracle, name):
+    prob, p = _setup(engine, name)
+    lr = feyntrop_amd.Learner(engine, p, init_seed=3)
+    lr.set_params(_rand_par(np.random.default_rng(5), out_scale=0.3))
+    Y = lr.forward()
+    Jr, r, _, _ = p.jtable()
+    with np.errstate(divide="ignore"):
+        w = Jr / r
+    t = lr.tables()
+    tr = LR.tables(Y, p.E, w, Jr, r)
+    n_states = 1 << p.E
+    used = np.arange(n_states) != n_states - 1    # a/lia of the full graph are never used
+    for k in ("lw", "lZ", "lratio"):
+        np.testing.assert_allclose(t[k], tr[k], rtol=1e-13, atol=0, err_msg=k)
+    for k in ("lia", "lrad", "ldiff"):
+        np.testing.assert_allclose(t[k][used], tr[k][used], rtol=1e-13, atol=1e-15, err_msg=k)
+    n, first, seed = 20000, 12345, 9
+    E = p.E
+    Xd = torch.zeros(n * E, dtype=torch.float64, device="cuda")
+    wd = torch.zeros(n, dtype=torch.float64, device="cuda")
+    pd = torch.zeros(n, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    lr.sample(first, n, seed, Xd.data_ptr(), wd.data_ptr(), pd.data_ptr())
+    X = Xd.cpu().numpy().reshape(n, E)
+    wt = wd.cpu().numpy()
+    path = pd.cpu().numpy().astype(np.uint64)
+    order = np.stack([((path >> np.uint64(4 * j)) & np.uint64(15)).astype(np.int64) for j in range(E)], 1)
+    U = LR.uniforms(oracle, seed, first, n, E)
+    Xr, wr, orr = LR.replay(t, E, U)
+    np.testing.assert_array_equal(order, orr)
+    np.testing.assert_allclose(X, Xr, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(wt, wr, rtol=1e-12)
+    np.testing.assert_allclose(wt[:4000], LR.weight_from_path(X[:4000], order[:4000], t, w, Jr, r), rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["double_box", "triple_ladder", "2L_3pt_tutorial"])
+def test_untrained_learner_is_plain_sampler(engine, name):
+    prob, p = _setup(engine, name)
+    lr = feyntrop_amd.Learner(engine, p, init_seed=1)
+    N = 2_000_000
+    m0 = engine.integrate(p, N, 4)
+    m1 = lr.integrate(N, 4)
+    assert m1.dropped == m0.dropped
+    for k in range(p.K):
+        for q in range(2 if p.info.path == 2 else 1):
+            assert m1.n[k][q] == m0.n[k][q]
+            np.testing.assert_allclose(m1.s1[k][q], m0.s1[k][q], rtol=1e-12, atol=1e-300)
+            np.testing.assert_allclose(m1.s2[k][q], m0.s2[k][q], rtol=1e-12, atol=1e-300)
+
+
+def _rel2(p, m):
+    r = p.finalize(m)
+    return r[0, 0, 1] ** 2 * m.n[0][0] / r[0, 0, 0] ** 2
+
+
+@pytest.mark.parametrize("name", ["triple_ladder", "double_box", "2L_3pt_tutorial", "cubic5"])
+def test_training_unbiased_and_lowers_variance(engine, name):
+    prob, p = _setup(engine, name)
+    lr = feyntrop_amd.Learner(engine, p, init_seed=1)
+    st = lr.train(steps=150, samples=1 << 18, seed=100, lr=3e-3)
+    assert np.all(np.isfinite(st[:, 0]))
+    N = 20_000_000
+    mp = engine.integrate(p, N, 21)
+    ml = lr.integrate(N, 22)
+    rp, rl = p.finalize(mp), p.finalize(ml)
+    for k in range(p.K):
+        for q in range(2 if p.info.path == 2 else 1):
+            a, da = rl[k, q]
+            b, db = rp[k, q]
+            assert abs(a - b) <= 3.0 * np.hypot(da, db) + 1e-300, (k, q, a, da, b, db)
+    gain = _rel2(p, mp) / _rel2(p, ml)
+    assert gain > 1.5, gain
+
+
+def test_learned_wheel_period_6zeta3(engine):
+    prob, p = _setup(engine, "wheel_k4")
+    lr = feyntrop_amd.Learner(engine, p, init_seed=2)
+    lr.train(steps=100, samples=1 << 18, seed=7, lr=3e-3)
+    m = lr.integrate(10**7, 3)
+    r = p.finalize(m)
+    z3 = 1.2020569031595942853997381615114
+    assert abs(r[0, 0, 0] - 6 * z3) <= 3 * r[0, 0, 1]
