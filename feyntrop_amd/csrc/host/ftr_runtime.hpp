@@ -88,5 +88,6 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
 void ensure_scratch(ftr_ctx* c, size_t bytes);
 void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
 Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit);
+size_t variant_lds(const Variant& v, int E);
 int fail(ftr_ctx* c, int code, const std::string& msg);
 extern thread_local std::string g_host_err;

@@ -33,6 +33,7 @@ constexpr size_t NPAR = OFF_B4 + NOUT;
 static_assert(NPAR == FTR_LEARN_NPARAMS, "parameter layout");
 constexpr int MAX_E = 16;          /* cut order packed 4 bits per step */
 constexpr double DMAX = 8.0;       /* |Delta| bound (cut-weight tilt) */
+constexpr int LEARN_BLOCK = 256;  /* one wave per SIMD per block: the full register budget per lane */
 constexpr double SMAX = 0.6;       /* |sigma| bound: a = r e^sigma < 2r keeps p/q of finite variance */
 
 struct MlpModule {
@@ -81,6 +82,7 @@ struct ftr_learner {
         *d_dZ = nullptr;
   double *d_lw = nullptr, *d_lZ = nullptr, *d_lratio = nullptr, *d_lia = nullptr, *d_lrad = nullptr,
          *d_ldiff = nullptr, *d_r = nullptr, *d_gC = nullptr, *d_gS = nullptr, *d_gsum = nullptr;
+  ftr::LQuad* d_lst = nullptr; /* per-state records read by the sampler */
   int adam_t = 0;
   bool tables_valid = false;
   std::vector<void*> allocs;
@@ -187,7 +189,7 @@ void build_tables(ftr_ctx* c, ftr_learner* l) {
   int E = l->E;
   double dmax = DMAX, smax = SMAX;
   void* args[] = {&l->d_Y, &E, &l->p->d_w, &l->p->d_jr, &l->d_r, &l->d_lw, &l->d_lZ, &l->d_lratio,
-                  &l->d_lia, &l->d_lrad, &l->d_ldiff, &dmax, &smax};
+                  &l->d_lia, &l->d_lrad, &l->d_ldiff, &dmax, &smax, &l->d_lst};
   launch(l->mlp.tables, (l->NT + 255) / 256, 1, 256, c->stream, args);
 }
 
@@ -212,6 +214,7 @@ ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
   L.a.N = N;
   L.a.S = G.S;
   L.a.partials = G.d_partials;
+  L.lst = l->d_lst;
   L.lw = l->d_lw;
   L.lZ = l->d_lZ;
   L.lratio = l->d_lratio;
@@ -276,7 +279,8 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->M = std::max<int>(32, (int)l->NT);
     l->mlp = load_mlp(c);
     l->var = p->var;
-    l->lds = p->lds;
+    l->var.block = LEARN_BLOCK;
+    l->lds = variant_lds(l->var, E);
     const hipModule_t gm = load_module_raw(c, full_source(p->prep, l->var, true));
     HIPCHK(hipModuleGetFunction(&l->f_lev, gm, "ftrl_lev"));
     HIPCHK(hipModuleGetFunction(&l->f_lgr, gm, "ftrl_lgr"));
@@ -311,6 +315,7 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->d_r = dalloc<double>(l, NT);
     l->d_gS = dalloc<double>(l, NT);
     l->d_gsum = dalloc<double>(l, 2);
+    l->d_lst = dalloc<ftr::LQuad>(l, NT);
     /* initial parameters: hidden layers uniform(+-sqrt(3/fan_in)), zero
      * biases, zero output layer (q = p_trop) */
     std::vector<float> par(NPAR, 0.0f);
@@ -553,8 +558,11 @@ int ftr_precompile_learned(const ftr_input* in, char* err, int errlen) {
     prep_from_input(pr, in);
     if (pr.g.E > MAX_E) throw Error(FTR_ERR_LIMIT, "the learned proposal supports at most 16 edges");
     code_object(ftr_embedded_mlp_source, nullptr);
-    for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024})
-      code_object(full_source(pr, choose_variant(pr, lim), true), nullptr);
+    for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
+      Variant v = choose_variant(pr, lim);
+      v.block = LEARN_BLOCK;
+      code_object(full_source(pr, v, true), nullptr);
+    }
   } catch (std::exception& e) {
     if (err && errlen > 0) {
       strncpy(err, e.what(), errlen - 1);

@@ -103,7 +103,7 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
   return out;
 }
 
-static size_t variant_lds(const Variant& v, int E) {
+size_t variant_lds(const Variant& v, int E) {
   size_t t = v.tmode == 0 ? ((size_t)25 << E) + 64 : (v.tmode == 1 ? ((size_t)8 << E) : 0);
   t = (t + 15) & ~(size_t)15;
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;

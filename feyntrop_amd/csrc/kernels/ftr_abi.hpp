@@ -68,8 +68,15 @@ struct BatchItem {
  * Training accumulators (doubles): gC[g*E+e] += c for every visit of g that
  * cut e; gS[g] += c (1 + a(g) log Y) for the radius drawn after g;
  * gsum[0] += c, gsum[1] += 1, with c = weight * |R_0|. */
+/* Per-state record of the learned tables, one 64-byte load per visited state:
+ * lZ(g), A(g) = lZ(g)/J(g) and the radial parameters lia, lrad, ldiff. */
+struct alignas(64) LQuad {
+  double lZ, A, lia, lrad, ldiff, pad0, pad1, pad2;
+};
+
 struct LearnLaunch {
   Launch a;
+  const LQuad* lst;
   const double* lw;
   const double* lZ;
   const double* lratio;

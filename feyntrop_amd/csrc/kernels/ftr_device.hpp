@@ -753,8 +753,94 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
 /* Learned proposal: the cut and the radius distributions come from the
  * learned tables; the exact FP64 ratio p_trop/q of the drawn path multiplies
  * the integrand, so E_q[R p/q] is the same integral.  Same 2E-1 draws per
- * sample as the plain sampler.  GRAD: also accumulate the forward-KL
- * statistics of the drawn paths. */
+ * sample as the plain sampler.
+ *
+ * One sample: X, log psi_tr, log G, the weight p_trop/q, the cut order (4
+ * bits per step) and, with KEEP_LY, the radial log-draws.  The E cut weights
+ * of the current state are loaded together (one memory latency per step,
+ * not one per candidate), and the running sum over them in ascending edge
+ * order selects the cut as the tropical sampler does (zero weights of
+ * absent edges leave the sum unchanged). */
+template <class G, class TV, bool KEEP_LY>
+__device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV& tv, double (&X)[G::E],
+                                   double& lpsi, double& lG, double& wgt, uint64_t& path, double (&lYs)[G::E]) {
+  constexpr int E = G::E;
+  uint32_t g = (1u << E) - 1u;
+  LQuad st = L.lst[g];
+  const uint32_t b0 = tv.mmL(g);
+  bool MM = b0 & 1;
+  int oL = (int)(b0 >> 1);
+  double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
+  path = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    X[e] = 0.0;
+    if (KEEP_LY) lYs[e] = 0.0;
+  }
+  for (int j = 0; j < E; j++) {
+    const double R = rng.uniform() * st.lZ;
+    const double* row = L.lw + (size_t)g * E;
+    double rv[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) rv[q] = row[q];
+    double T = 0.0, lsel = 0.0, lastv = 0.0;
+    int e = 0, last = 0;
+    bool found = false;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      T += rv[q];
+      if ((g >> q) & 1u) {
+        last = q;
+        lastv = rv[q];
+      }
+      const bool hit = !found && T > R;
+      if (hit) {
+        e = q;
+        lsel = rv[q];
+      }
+      found = found || hit;
+    }
+    if (!found) {
+      e = last;
+      lsel = lastv;
+    }
+    const uint32_t c = g ^ (1u << e);
+    ratio *= st.A * (tv.w(c) / lsel);
+    st = L.lst[c];
+    const uint32_t b = tv.mmL(c);
+    const bool mm = b & 1;
+    const int Lc = (int)(b >> 1);
+    if (MM && !mm) lGv = lX;
+    MM = mm;
+#pragma unroll
+    for (int k = 0; k < E; k++)
+      if (k == e) X[k] = XX;
+    if (Lc < oL) {
+      lF += lX;
+      oL = Lc;
+    }
+    path |= (uint64_t)e << (4 * j);
+    if (j < E - 1) {
+      const double U = rng.uniform();
+      const double lY = st.lia * log(1. - U);
+      lX += lY;
+      XX = exp(lX);
+      lrad += st.lrad + st.ldiff * lY;
+      if (KEEP_LY) {
+#pragma unroll
+        for (int k = 0; k < E; k++)
+          if (k == j) lYs[k] = lY;
+      }
+    }
+    g = c;
+  }
+  lpsi = lF;
+  lG = lGv;
+  wgt = ratio * exp(lrad);
+}
+
+/* Integration with the learned proposal; GRAD also accumulates the
+ * forward-KL statistics of the drawn paths (ftr_abi.hpp LearnLaunch). */
 template <class G, int K, int MODE, int TMODE, bool GRAD>
 __device__ inline void learn_body(const LearnLaunch& L) {
   extern __shared__ __align__(16) char lds[];
@@ -775,62 +861,9 @@ __device__ inline void learn_body(const LearnLaunch& L) {
 #pragma unroll
   for (int k = 0; k < (GRAD ? E : 1); k++) gc0[k] = gs0[k] = 0.0;
   for (uint64_t i = i0; i < i1; i++) {
-    double X[E];
-    uint32_t g = (1u << E) - 1u;
-    const uint32_t b0 = tv.mmL(g);
-    bool MM = b0 & 1;
-    int oL = (int)(b0 >> 1);
-    double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
-    uint64_t path = 0;
-    double lYs[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      X[e] = 0.0;
-      lYs[e] = 0.0;
-    }
-    for (int j = 0; j < E; j++) {
-      const double R = rng.uniform() * L.lZ[g];
-      const double* row = L.lw + (size_t)g * E;
-      double T = 0.0;
-      int e = 0;
-      uint32_t m = g;
-      while (m) {
-        const int q = __builtin_ctz(m);
-        m &= m - 1;
-        T += row[q];
-        e = q;
-        if (T > R) break;
-      }
-      ratio *= L.lratio[(size_t)g * E + e];
-      const uint32_t c = g ^ (1u << e);
-      const uint32_t b = tv.mmL(c);
-      const bool mm = b & 1;
-      const int Lc = (int)(b >> 1);
-      if (MM && !mm) lGv = lX;
-      MM = mm;
-#pragma unroll
-      for (int k = 0; k < E; k++)
-        if (k == e) X[k] = XX;
-      if (Lc < oL) {
-        lF += lX;
-        oL = Lc;
-      }
-      if (GRAD) path |= (uint64_t)e << (4 * j);
-      if (j < E - 1) {
-        const double U = rng.uniform();
-        const double lY = L.lia[c] * log(1. - U);
-        lX += lY;
-        XX = exp(lX);
-        lrad += L.lrad[c] + L.ldiff[c] * lY;
-        if (GRAD) {
-#pragma unroll
-          for (int k = 0; k < E; k++)
-            if (k == j) lYs[k] = lY;
-        }
-      }
-      g = c;
-    }
-    const double wgt = ratio * exp(lrad);
+    double X[E], lYs[E], lF, lGv, wgt;
+    uint64_t path;
+    learn_point<G, TableView<TMODE>, GRAD>(L, rng, tv, X, lF, lGv, wgt, path, lYs);
     cplx out[K];
     if (!integrand<G, K, MODE>(p, X, log_scale(p, lF, lGv), out)) {
       acc[1] += 1.0;
@@ -904,50 +937,25 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   }
 }
 
-/* Learned sampler only (parity): X, the ratio weight and the cut order
- * (4 bits per step) of samples [first, first+n) of the stream. */
+/* Learned sampler only (parity): X, the weight p/q and the cut order of
+ * samples [first, first+n) of the stream. */
 template <class G, int TMODE>
 __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, double* wout, uint64_t* pout) {
   extern __shared__ __align__(16) char lds[];
   constexpr int E = G::E;
   const Launch& a = L.a;
   TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
-  (void)tv;
   Xoshiro rng = lane_stream(a, blockIdx.x);
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
   const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
   for (uint64_t i = i0; i < i1; i++) {
-    uint32_t g = (1u << E) - 1u;
-    double lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
-    uint64_t path = 0;
-    for (int j = 0; j < E; j++) {
-      const double R = rng.uniform() * L.lZ[g];
-      const double* row = L.lw + (size_t)g * E;
-      double T = 0.0;
-      int e = 0;
-      uint32_t m = g;
-      while (m) {
-        const int q = __builtin_ctz(m);
-        m &= m - 1;
-        T += row[q];
-        e = q;
-        if (T > R) break;
-      }
-      ratio *= L.lratio[(size_t)g * E + e];
-      const uint32_t c = g ^ (1u << e);
-      Xout[i * E + e] = XX;
-      path |= (uint64_t)e << (4 * j);
-      if (j < E - 1) {
-        const double U = rng.uniform();
-        const double lY = L.lia[c] * log(1. - U);
-        lX += lY;
-        XX = exp(lX);
-        lrad += L.lrad[c] + L.ldiff[c] * lY;
-      }
-      g = c;
-    }
-    wout[i] = ratio * exp(lrad);
+    double X[E], lYs[E], lF, lGv, wgt;
+    uint64_t path;
+    learn_point<G, TableView<TMODE>, false>(L, rng, tv, X, lF, lGv, wgt, path, lYs);
+#pragma unroll
+    for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
+    wout[i] = wgt;
     pout[i] = path;
   }
 }

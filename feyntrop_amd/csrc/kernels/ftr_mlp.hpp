@@ -167,7 +167,7 @@ extern "C" __global__ void mlp_to_bf16(const float* src, unsigned short* dst, ui
  * Delta = dmax tanh(y/dmax), sigma = smax tanh(y/smax). */
 extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double* w, const double* jr,
                                             const double* r, double* lw, double* lZ, double* lratio, double* lia,
-                                            double* lrad, double* ldiff, double dmax, double smax) {
+                                            double* lrad, double* ldiff, double dmax, double smax, double* lst) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (1u << E)) return;
   const float* y = Y + (size_t)g * 32;
@@ -188,6 +188,14 @@ extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double*
   lia[g] = 1.0 / aa;
   lrad[g] = -sg;
   ldiff[g] = r[g] - aa;
+  /* the sampler's per-state record (ftr_abi.hpp LQuad) */
+  double* q = lst + (size_t)g * 8;
+  q[0] = Z;
+  q[1] = Z / jr[g];
+  q[2] = 1.0 / aa;
+  q[3] = -sg;
+  q[4] = r[g] - aa;
+  q[5] = q[6] = q[7] = 0.0;
 }
 
 /* dL/dY of the forward KL  L = -sum_paths c log q(path) / sum c  from the
