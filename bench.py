@@ -125,6 +125,62 @@ def run_tuned(eng, name, steps=2):
             "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
 
 
+def _rel_err(p, m):
+    r = p.finalize(m)
+    I0 = complex(r[0, 0, 0], r[0, 1, 0])
+    return float(np.hypot(r[0, 0, 1], r[0, 1, 1]) / abs(I0)), I0, r
+
+
+def run_learned(eng, name="triple_ladder", steps=400, batch=1 << 18, lr=3e-3):
+    """Learned proposal against the plain sampler on the 3-loop ladder: the
+    network is trained on steps x batch (about 1e8) proposal samples, plain
+    sampling judges it (select), then both integrate N = 1e8 samples.
+    time-to-1e-3 = (rel. err at N / 1e-3)^2 x wall time at N (+ training for
+    the learned proposal); equal wall-clock: the learned error scaled to the
+    plain run's wall time."""
+    import feyntrop_amd
+    from feyntrop_amd import workloads as W
+    prob = W.all_problems()[name]()
+    N = int(prob["N"])
+    p = eng.prepare(prob)
+    eng.integrate(p, 1_000_000, 1)
+    t0 = time.perf_counter()
+    mp = eng.integrate(p, N, 11)
+    tp = time.perf_counter() - t0
+    rel_p, Ip, rp = _rel_err(p, mp)
+    L = feyntrop_amd.Learner(eng, p, init_seed=1)
+    t0 = time.perf_counter()
+    st = L.train(steps, batch, seed=100, lr=lr)
+    t_train = time.perf_counter() - t0
+    use, eff_p, eff_l = L.select(n_probe=4_000_000, seed=7)
+    L.integrate(1_000_000, 2)
+    t0 = time.perf_counter()
+    ml = L.integrate(N, 12)
+    tl = time.perf_counter() - t0
+    rel_l, Il, rl = _rel_err(p, ml)
+    t13_p = (rel_p / 1e-3) ** 2 * tp
+    t13_l = (rel_l / 1e-3) ** 2 * tl + t_train
+    z = [abs(rl[k, q, 0] - rp[k, q, 0]) / max(np.hypot(rl[k, q, 1], rp[k, q, 1]), 1e-300)
+         for k in range(p.K) for q in range(2)]
+    fps = flops_per_sample(p.info)
+    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}, N={N:.0e} each, lambda={prob['lambda']}",
+            "network": "MLP 32-128-128-128-32 SiLU, BF16 MFMA (f32 accumulate), Adam",
+            "train_samples": steps * batch, "train_s": t_train,
+            "train_rel2_first_last": [float(st[0, 0]), float(st[-1, 0])],
+            "selected": "learned" if use else "plain",
+            "plain": {"samples_per_s": N / tp, "rel_err": rel_p, "time_to_1e-3_s": t13_p,
+                      "kernel_ms": mp.kernel_ms,
+                      "roofline_frac": fps * N / (mp.kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+            "learned": {"samples_per_s": N / tl, "rel_err": rel_l, "time_to_1e-3_s": t13_l,
+                        "kernel_ms": ml.kernel_ms,
+                        "roofline_frac": fps * N / (ml.kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+            "variance_gain": (rel_p ** 2) / (rel_l ** 2),
+            "speedup_time_to_1e-3": t13_p / t13_l,
+            "rel_err_at_equal_wallclock": {"plain": rel_p, "learned": rel_l * np.sqrt((tl + t_train) / tp)},
+            "max_sigma_plain_vs_learned": float(max(z)),
+            "eps0": {"plain": [Ip.real, Ip.imag], "learned": [Il.real, Il.imag]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +192,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--scan-points", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -190,6 +247,11 @@ def main():
                      "kernel": "ftrk_int", "flops_per_sample": fps, "kernel_ms_mean": mean_kms},
         "learned": None,
     }
+    if not args.no_learned:
+        try:
+            line["learned"] = run_learned(eng)
+        except Exception as e:  # reported, not fatal for the main line
+            line["learned"] = {"error": str(e)[:300]}
     if not args.no_extras:
         extras = {}
         for key, fn in (("scan", lambda: run_scan(eng, args.scan_points)),

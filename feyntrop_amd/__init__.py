@@ -107,6 +107,9 @@ def lib():
     L.ftr_learner_integrate.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts),
                                         C.POINTER(ftr_moments)]
     L.ftr_learner_sample.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp, vp]
+    L.ftr_learner_set_objective.argtypes = [vp, C.c_int]
+    L.ftr_learner_select.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -369,6 +372,18 @@ class Learner:
         self._chk(lib().ftr_learner_integrate(self.engine.h, self.h, int(N), int(seed), C.byref(opts),
                                               C.byref(m)))
         return m
+
+    def set_objective(self, objective):
+        """0: forward KL (default), 1: variance of the estimator."""
+        self._chk(lib().ftr_learner_set_objective(self.h, int(objective)))
+
+    def select(self, n_probe=4_000_000, seed=12345):
+        """Plain sampling judges the network: returns (use_learned, eff_plain,
+        eff_learned); integrate() then uses the better proposal."""
+        u, ep, el = C.c_int(), C.c_double(), C.c_double()
+        self._chk(lib().ftr_learner_select(self.engine.h, self.h, int(n_probe), int(seed), C.byref(u),
+                                           C.byref(ep), C.byref(el)))
+        return bool(u.value), ep.value, el.value
 
     def sample(self, first, n, seed, X_ptr, w_ptr, path_ptr):
         """Device pointers of n*E doubles, n doubles and n uint64."""

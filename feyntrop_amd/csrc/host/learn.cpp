@@ -33,6 +33,8 @@ constexpr size_t NPAR = OFF_B4 + NOUT;
 static_assert(NPAR == FTR_LEARN_NPARAMS, "parameter layout");
 constexpr int MAX_E = 16;          /* cut order packed 4 bits per step */
 constexpr double DMAX = 8.0;       /* |Delta| bound (cut-weight tilt) */
+constexpr int HIST_E = 10;         /* histogram in LDS up to this E (ftr_device.hpp) */
+constexpr size_t hist_len(int E) { return ((size_t)E + 1) * ((size_t)1 << E) + 2; }
 constexpr int LEARN_BLOCK = 256;  /* one wave per SIMD per block: the full register budget per lane */
 constexpr double SMAX = 0.6;       /* |sigma| bound: a = r e^sigma < 2r keeps p/q of finite variance */
 
@@ -75,7 +77,13 @@ struct ftr_learner {
   uint32_t NT = 0;
   int M = 0; /* network rows: max(32, 2^E) */
   MlpModule mlp;
-  hipFunction_t f_lev = nullptr, f_lgr = nullptr, f_lsm = nullptr;
+  hipFunction_t f_lev = nullptr, f_lgr = nullptr, f_lsm = nullptr, f_acc = nullptr, f_hst = nullptr,
+                f_hrd = nullptr;
+  double* d_part = nullptr; /* per-block histograms (E <= HIST_E) */
+  int hist_blocks = 0;
+  double* d_gbuf = nullptr; /* training record (see ftr_abi.hpp LearnLaunch) */
+  uint64_t* d_gpath = nullptr;
+  uint64_t rec_cap = 0;     /* samples the record holds */
   float *d_par = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
   uint16_t *d_parb = nullptr, *d_X0b = nullptr, *d_H1 = nullptr, *d_H2 = nullptr, *d_H3 = nullptr;
   float *d_Z1 = nullptr, *d_Z2 = nullptr, *d_Z3 = nullptr, *d_Y = nullptr, *d_dY = nullptr, *d_dH = nullptr,
@@ -85,6 +93,8 @@ struct ftr_learner {
   ftr::LQuad* d_lst = nullptr; /* per-state records read by the sampler */
   int adam_t = 0;
   bool tables_valid = false;
+  bool use_plain = false; /* set by ftr_learner_select when the plain sampler wins */
+  int objective = 0;      /* 0 forward KL, 1 variance */
   std::vector<void*> allocs;
 };
 
@@ -105,13 +115,13 @@ void launch(hipFunction_t f, unsigned gx, unsigned gy, unsigned block, hipStream
 /* C = act(A B + bias), operands addressed by strides (see ftr_mlp.hpp) */
 void gemm(ftr_ctx* c, ftr_learner* l, bool a_float, const void* A, uint64_t sai, uint64_t sak, const uint16_t* B,
           uint64_t sbk, uint64_t sbj, const float* bias, float* C, uint64_t ldc, uint16_t* Cb, uint64_t ldcb, int M,
-          int N, int K, int act) {
+          int N, int K, int act, bool c_zeroed = false) {
   if (M % 32 || N % 32 || K % 8) throw Error(FTR_ERR_INVALID, "network GEMM shape not tiled");
   /* long reductions (weight gradients over all states) split K over blocks
-   * that add into C atomically; C must be zero then */
+   * that add into C atomically when the caller has zeroed C */
   int splits = 1, kc = K;
-  if (K >= 1024 && !Cb && !bias) {
-    kc = 256;
+  if (c_zeroed && K >= 256 && !Cb && !bias && !act) {
+    kc = 64;
     splits = (K + kc - 1) / kc;
   }
   void* args[] = {(void*)&A, &sai, &sak, (void*)&B, &sbk, &sbj, (void*)&bias, &C, &ldc, &Cb, &ldcb, &K, &act, &kc};
@@ -148,7 +158,7 @@ void forward(ftr_ctx* c, ftr_learner* l) {
 
 /* db += column sums of dZ (db zeroed beforehand) */
 void colsum(ftr_ctx* c, ftr_learner* l, const float* dZ, float* db, int rows, int cols) {
-  int chunk = 256;
+  int chunk = 32;
   void* args[] = {(void*)&dZ, &db, &rows, &cols, &chunk};
   launch(l->mlp.colsum, (unsigned)((cols + 63) / 64), (unsigned)((rows + chunk - 1) / chunk), 64, c->stream, args);
 }
@@ -165,23 +175,23 @@ void backward(ftr_ctx* c, ftr_learner* l) {
   const int M = l->M;
   fill_f(c, l, G, NPAR, 0.0f); /* split-K GEMMs and column sums accumulate */
   /* layer 4: dW4 = dY^T H3, db4 = colsum dY, dH3 = dY W4 */
-  gemm(c, l, true, l->d_dY, 1, NOUT, l->d_H3, H, 1, nullptr, G + OFF_W4, H, nullptr, 0, NOUT, H, M, 0);
+  gemm(c, l, true, l->d_dY, 1, NOUT, l->d_H3, H, 1, nullptr, G + OFF_W4, H, nullptr, 0, NOUT, H, M, 0, true);
   colsum(c, l, l->d_dY, G + OFF_B4, M, NOUT);
   gemm(c, l, true, l->d_dY, NOUT, 1, W + OFF_W4, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, NOUT, 0);
   const uint64_t n = (uint64_t)M * H;
   /* layer 3 */
   silu_bwd(c, l, l->d_dH, l->d_Z3, l->d_dZ, n);
-  gemm(c, l, true, l->d_dZ, 1, H, l->d_H2, H, 1, nullptr, G + OFF_W3, H, nullptr, 0, H, H, M, 0);
+  gemm(c, l, true, l->d_dZ, 1, H, l->d_H2, H, 1, nullptr, G + OFF_W3, H, nullptr, 0, H, H, M, 0, true);
   colsum(c, l, l->d_dZ, G + OFF_B3, M, H);
   gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W3, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 0);
   /* layer 2 */
   silu_bwd(c, l, l->d_dH, l->d_Z2, l->d_dZ, n);
-  gemm(c, l, true, l->d_dZ, 1, H, l->d_H1, H, 1, nullptr, G + OFF_W2, H, nullptr, 0, H, H, M, 0);
+  gemm(c, l, true, l->d_dZ, 1, H, l->d_H1, H, 1, nullptr, G + OFF_W2, H, nullptr, 0, H, H, M, 0, true);
   colsum(c, l, l->d_dZ, G + OFF_B2, M, H);
   gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W2, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 0);
   /* layer 1 */
   silu_bwd(c, l, l->d_dH, l->d_Z1, l->d_dZ, n);
-  gemm(c, l, true, l->d_dZ, 1, H, l->d_X0b, NIN, 1, nullptr, G + OFF_W1, NIN, nullptr, 0, H, NIN, M, 0);
+  gemm(c, l, true, l->d_dZ, 1, H, l->d_X0b, NIN, 1, nullptr, G + OFF_W1, NIN, nullptr, 0, H, NIN, M, 0, true);
   colsum(c, l, l->d_dZ, G + OFF_B1, M, H);
 }
 
@@ -224,6 +234,9 @@ ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
   L.gC = l->d_gC;
   L.gS = l->d_gS;
   L.gsum = l->d_gsum;
+  L.gbuf = l->d_gbuf;
+  L.gpath = l->d_gpath;
+  L.objective = l->objective;
   return L;
 }
 
@@ -285,6 +298,17 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     HIPCHK(hipModuleGetFunction(&l->f_lev, gm, "ftrl_lev"));
     HIPCHK(hipModuleGetFunction(&l->f_lgr, gm, "ftrl_lgr"));
     HIPCHK(hipModuleGetFunction(&l->f_lsm, gm, "ftrl_lsm"));
+    HIPCHK(hipModuleGetFunction(&l->f_acc, gm, "ftrl_acc"));
+    HIPCHK(hipModuleGetFunction(&l->f_hst, gm, "ftrl_hst"));
+    HIPCHK(hipModuleGetFunction(&l->f_hrd, gm, "ftrl_hrd"));
+    if (E <= HIST_E) {
+      /* one 1024-thread block per CU, each with the whole histogram in LDS */
+      const size_t hb = hist_len(E) * 8;
+      if (hb > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void*)l->f_hst, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hb));
+      l->hist_blocks = c->prop.multiProcessorCount;
+      l->d_part = dalloc<double>(l, (size_t)l->hist_blocks * hist_len(E));
+    }
     if (l->lds > 64 * 1024)
       for (hipFunction_t f : {l->f_lev, l->f_lgr, l->f_lsm})
         (void)hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l->lds);
@@ -328,7 +352,18 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     init(OFF_W1, H, NIN);
     init(OFF_W2, H, H);
     init(OFF_W3, H, H);
-    /* inputs per state g: its edge bits, popcount/E, 1, r(g)/E */
+    /* inputs per state g: its edge bits, popcount/E, 1, r(g)/E, then the
+     * kinematics: the scalar products among the vertices with external
+     * momentum (upper triangle, row-major, at most 13), scaled by
+     * max(1, their largest magnitude) */
+    std::vector<double> kin;
+    const std::vector<int>& ext = p->prep.ext;
+    const int V = p->prep.g.V;
+    for (size_t a = 0; a < ext.size(); a++)
+      for (size_t b = a; b < ext.size(); b++)
+        if (kin.size() < (size_t)(NIN - MAX_E - 3)) kin.push_back(p->prep.P[(size_t)ext[a] * V + ext[b]]);
+    double ks = 1.0;
+    for (double v : kin) ks = std::max(ks, std::fabs(v));
     std::vector<float> X0((size_t)M * NIN, 0.0f);
     for (uint32_t g = 0; g < l->NT; g++) {
       float* x = &X0[(size_t)g * NIN];
@@ -336,6 +371,7 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
       x[MAX_E] = (float)__builtin_popcount(g) / (float)E;
       x[MAX_E + 1] = 1.0f;
       x[MAX_E + 2] = (float)(p->prep.r[g] / E);
+      for (size_t k = 0; k < kin.size(); k++) x[MAX_E + 3 + k] = (float)(kin[k] / ks);
     }
     std::vector<float> zeros(NPAR, 0.0f);
     float* d_x0f = nullptr;
@@ -365,6 +401,8 @@ void ftr_learner_free(ftr_learner* l) {
   if (!l) return;
   (void)hipSetDevice(l->device);
   for (void* p : l->allocs) (void)hipFree(p);
+  if (l->d_gbuf) (void)hipFree(l->d_gbuf);
+  if (l->d_gpath) (void)hipFree(l->d_gpath);
   delete l;
 }
 
@@ -427,18 +465,47 @@ int ftr_learner_train(ftr_ctx* c, ftr_learner* l, int steps, uint64_t samples, u
     HIPCHK(hipSetDevice(c->device));
     const int K = p->prep.K, NQ = 2 + 4 * K;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    if (samples > l->rec_cap) {
+      if (l->d_gbuf) HIPCHK(hipFree(l->d_gbuf));
+      if (l->d_gpath) HIPCHK(hipFree(l->d_gpath));
+      l->d_gbuf = nullptr;
+      l->d_gpath = nullptr;
+      l->rec_cap = 0;
+      HIPCHK(hipMalloc((void**)&l->d_gbuf, (size_t)samples * l->E * 8));
+      HIPCHK(hipMalloc((void**)&l->d_gpath, (size_t)samples * 8));
+      l->rec_cap = samples;
+    }
+    const int E = l->E;
+    const size_t acc_lds = (size_t)(2 + 2 * E + E * E + E * (E - 1) / 2 * (1 + E)) * 8;
     for (int st = 0; st < steps; st++) {
       forward(c, l);
       build_tables(c, l);
-      fill_d(c, l, l->d_gC, (uint64_t)l->NT * l->E, 0.0);
-      fill_d(c, l, l->d_gS, l->NT, 0.0);
-      fill_d(c, l, l->d_gsum, 2, 0.0);
+      if (E > HIST_E) { /* the atomic accumulation adds into them */
+        fill_d(c, l, l->d_gC, (uint64_t)l->NT * l->E, 0.0);
+        fill_d(c, l, l->d_gS, l->NT, 0.0);
+        fill_d(c, l, l->d_gsum, 2, 0.0);
+      }
       Geometry G = setup_stream(p, 0, samples, seed + (uint64_t)st, 0, l->var.block, NQ, l->f_lgr);
       ftr::LearnLaunch L = learn_launch(l, G, samples);
       void* args[] = {&L};
       HIPCHK(hipEventRecord(c->ev0, c->stream));
       HIPCHK(hipModuleLaunchKernel(l->f_lgr, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
                                    c->stream, args, nullptr));
+      if (E <= HIST_E) {
+        uint64_t n = samples;
+        void* hargs[] = {&L, &n, &l->d_part};
+        HIPCHK(hipModuleLaunchKernel(l->f_hst, (unsigned)l->hist_blocks, 1, 1, 1024, 1, 1,
+                                     (unsigned)(hist_len(E) * 8), c->stream, hargs, nullptr));
+        int nb = l->hist_blocks;
+        void* rargs[] = {&L, &l->d_part, &nb};
+        HIPCHK(hipModuleLaunchKernel(l->f_hrd, (unsigned)((hist_len(E) + 255) / 256), 1, 1, 256, 1, 1, 0,
+                                     c->stream, rargs, nullptr));
+      } else {
+        uint64_t n = samples;
+        void* aargs[] = {&L, &n};
+        HIPCHK(hipModuleLaunchKernel(l->f_acc, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1,
+                                     (unsigned)acc_lds, c->stream, aargs, nullptr));
+      }
       HIPCHK(hipEventRecord(c->ev1, c->stream));
       fill_f(c, l, l->d_dY, (uint64_t)l->M * NOUT, 0.0f);
       {
@@ -507,6 +574,7 @@ int ftr_learner_integrate(ftr_ctx* c, ftr_learner* l, uint64_t N, uint64_t seed,
     const int K = p->prep.K, NQ = 2 + 4 * K;
     memset(out, 0, sizeof *out);
     out->K = K;
+    if (l->use_plain) return ftr_integrate(c, p, N, seed, opts, out);
     if (opts && opts->block && opts->block != l->var.block)
       throw Error(FTR_ERR_INVALID, "block size differs from the compiled variant");
     ensure_tables(c, l);
@@ -529,6 +597,44 @@ int ftr_learner_integrate(ftr_ctx* c, ftr_learner* l, uint64_t N, uint64_t seed,
   } catch (std::exception& e) {
     return fail_exc(c, e);
   }
+  return FTR_OK;
+}
+
+int ftr_learner_set_objective(ftr_learner* l, int objective) {
+  if (!l || objective < 0 || objective > 1) return FTR_ERR_INVALID;
+  l->objective = objective;
+  return FTR_OK;
+}
+
+int ftr_learner_select(ftr_ctx* c, ftr_learner* l, uint64_t n_probe, uint64_t seed, int* use_learned,
+                       double* eff_plain, double* eff_learned) {
+  if (!c || !l || n_probe < 2) return FTR_ERR_INVALID;
+  ftr_problem* p = l->p;
+  /* efficiency = relative second moment of the eps^0 coefficient x kernel
+   * time per sample (lower is better) */
+  auto eff = [&](const ftr_moments& m) {
+    double s1 = m.s1[0][0], s2 = m.s2[0][0];
+    double n = m.n[0][0];
+    double v = s2 / n - (s1 / n) * (s1 / n);
+    if (p->prep.path == 2) {
+      const double t1 = m.s1[0][1], t2 = m.s2[0][1];
+      v += t2 / n - (t1 / n) * (t1 / n);
+      s1 = std::hypot(s1, t1);
+    }
+    const double mean2 = (s1 / n) * (s1 / n);
+    return mean2 > 0 ? v / mean2 * (m.kernel_ms / (double)n_probe) : HUGE_VAL;
+  };
+  ftr_moments mp, ml;
+  l->use_plain = false;
+  int rc = ftr_integrate(c, p, n_probe, seed, nullptr, &mp);
+  if (rc) return rc;
+  rc = ftr_learner_integrate(c, l, n_probe, seed + 1, nullptr, &ml);
+  if (rc) return rc;
+  const double ep = eff(mp), el = eff(ml);
+  l->use_plain = !(el < ep);
+  if (use_learned) *use_learned = l->use_plain ? 0 : 1;
+  if (eff_plain) *eff_plain = ep;
+  if (eff_learned) *eff_learned = el;
   return FTR_OK;
 }
 

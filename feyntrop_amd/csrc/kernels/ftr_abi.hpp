@@ -66,8 +66,8 @@ struct BatchItem {
  *                 arriving at g (plain sampling: a = r)
  *   lrad[g]       log(r/a) = -sigma(g);   ldiff[g]  r(g) - a(g)
  * Training accumulators (doubles): gC[g*E+e] += c for every visit of g that
- * cut e; gS[g] += c (1 + a(g) log Y) for the radius drawn after g;
- * gsum[0] += c, gsum[1] += 1, with c = weight * |R_0|. */
+ * cut e; gS[g] += c (1 + a(g) log Y) for the radius drawn after arriving at
+ * g; gsum[0] += c, gsum[1] += 1, with c = weight * |R_0|. */
 /* Per-state record of the learned tables, one 64-byte load per visited state:
  * lZ(g), A(g) = lZ(g)/J(g) and the radial parameters lia, lrad, ldiff. */
 struct alignas(64) LQuad {
@@ -86,6 +86,9 @@ struct LearnLaunch {
   double* gC;
   double* gS;
   double* gsum;
+  double* gbuf;      /* training record: c (N), then lY_j (N each, j < E-1) */
+  int objective;     /* 0: forward KL, c = w|f|;  1: variance, c = (w|f|)^2 */
+  uint64_t* gpath;   /* training record: cut order per sample */
 };
 
 }  // namespace ftr

@@ -756,14 +756,16 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
  * sample as the plain sampler.
  *
  * One sample: X, log psi_tr, log G, the weight p_trop/q, the cut order (4
- * bits per step) and, with KEEP_LY, the radial log-draws.  The E cut weights
+ * bits per step) and, with KEEP_LY, the radial log-draws (stored to ly_out
+ * with stride ly_stride).  The E cut weights
  * of the current state are loaded together (one memory latency per step,
  * not one per candidate), and the running sum over them in ascending edge
  * order selects the cut as the tropical sampler does (zero weights of
  * absent edges leave the sum unchanged). */
 template <class G, class TV, bool KEEP_LY>
 __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV& tv, double (&X)[G::E],
-                                   double& lpsi, double& lG, double& wgt, uint64_t& path, double (&lYs)[G::E]) {
+                                   double& lpsi, double& lG, double& wgt, uint64_t& path, double* ly_out,
+                                   uint64_t ly_stride) {
   constexpr int E = G::E;
   uint32_t g = (1u << E) - 1u;
   LQuad st = L.lst[g];
@@ -773,10 +775,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV&
   double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
   path = 0;
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    X[e] = 0.0;
-    if (KEEP_LY) lYs[e] = 0.0;
-  }
+  for (int e = 0; e < E; e++) X[e] = 0.0;
   for (int j = 0; j < E; j++) {
     const double R = rng.uniform() * st.lZ;
     const double* row = L.lw + (size_t)g * E;
@@ -826,11 +825,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV&
       lX += lY;
       XX = exp(lX);
       lrad += st.lrad + st.ldiff * lY;
-      if (KEEP_LY) {
-#pragma unroll
-        for (int k = 0; k < E; k++)
-          if (k == j) lYs[k] = lY;
-      }
+      if (KEEP_LY) ly_out[(size_t)j * ly_stride] = lY;
     }
     g = c;
   }
@@ -839,8 +834,11 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV&
   wgt = ratio * exp(lrad);
 }
 
-/* Integration with the learned proposal; GRAD also accumulates the
- * forward-KL statistics of the drawn paths (ftr_abi.hpp LearnLaunch). */
+/* Integration with the learned proposal.  GRAD (training) also records
+ * every sample's path, c = |f| p_trop/q or its square (objective; 0 when
+ * dropped) and radial
+ * log-draws for learn_acc_body: gbuf[i] = c, gbuf[(1+j) N + i] = lY_j,
+ * gpath[i] = cut order. */
 template <class G, int K, int MODE, int TMODE, bool GRAD>
 __device__ inline void learn_body(const LearnLaunch& L) {
   extern __shared__ __align__(16) char lds[];
@@ -857,16 +855,16 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   double acc[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) acc[q] = 0.0;
-  double gc0[GRAD ? E : 1], gs0[GRAD ? E : 1], gsum0 = 0.0, gsum1 = 0.0;
-#pragma unroll
-  for (int k = 0; k < (GRAD ? E : 1); k++) gc0[k] = gs0[k] = 0.0;
   for (uint64_t i = i0; i < i1; i++) {
-    double X[E], lYs[E], lF, lGv, wgt;
+    double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, TableView<TMODE>, GRAD>(L, rng, tv, X, lF, lGv, wgt, path, lYs);
+    learn_point<G, TableView<TMODE>, GRAD>(L, rng, tv, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr,
+                                           a.N);
+    if (GRAD) L.gpath[i] = path;
     cplx out[K];
     if (!integrand<G, K, MODE>(p, X, log_scale(p, lF, lGv), out)) {
       acc[1] += 1.0;
+      if (GRAD) L.gbuf[i] = 0.0;
       continue;
     }
     acc[0] += 1.0;
@@ -879,48 +877,8 @@ __device__ inline void learn_body(const LearnLaunch& L) {
       acc[5 + 4 * k] += im * im;
     }
     if (GRAD) {
-      /* the first cut (from the full graph) and the radius after it hit E
-       * addresses shared by every sample: accumulated per lane and added
-       * once per wave; deeper states spread over many addresses */
-      const double cs = wgt * sqrt(out[0].re * out[0].re + out[0].im * out[0].im);
-      const int e0 = (int)(path & 15u);
-      const uint32_t c0 = ((1u << E) - 1u) ^ (1u << e0);
-      const double s0v = (E > 1) ? cs * (1.0 + lYs[0] / L.lia[c0]) : 0.0;
-#pragma unroll
-      for (int k = 0; k < E; k++)
-        if (k == e0) {
-          gc0[k] += cs;
-          gs0[k] += s0v;
-        }
-      gsum0 += cs;
-      gsum1 += 1.0;
-      uint32_t gg = c0;
-#pragma unroll
-      for (int j = 1; j < E; j++) {
-        const int e = (int)((path >> (4 * j)) & 15u);
-        unsafeAtomicAdd(&L.gC[(size_t)gg * E + e], cs);
-        const uint32_t c = gg ^ (1u << e);
-        if (j < E - 1) unsafeAtomicAdd(&L.gS[c], cs * (1.0 + lYs[j] / L.lia[c]));
-        gg = c;
-      }
-    }
-  }
-  if (GRAD) {
-    const int lid0 = threadIdx.x & 63;
-    const uint32_t full = (1u << E) - 1u;
-#pragma unroll
-    for (int k = 0; k < E; k++) {
-      const double vc = wave_sum(gc0[k]);
-      const double vs = wave_sum(gs0[k]);
-      if (lid0 == 0 && vc != 0.0) {
-        unsafeAtomicAdd(&L.gC[(size_t)full * E + k], vc);
-        if (E > 1) unsafeAtomicAdd(&L.gS[full ^ (1u << k)], vs);
-      }
-    }
-    const double t0 = wave_sum(gsum0), t1 = wave_sum(gsum1);
-    if (lid0 == 0) {
-      unsafeAtomicAdd(&L.gsum[0], t0);
-      unsafeAtomicAdd(&L.gsum[1], t1);
+      const double cf = wgt * sqrt(out[0].re * out[0].re + out[0].im * out[0].im);
+      L.gbuf[i] = L.objective ? cf * cf : cf;
     }
   }
   const int wid = threadIdx.x >> 6, lid = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -937,6 +895,173 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   }
 }
 
+/* Forward-KL statistics of the recorded samples (one thread per sample):
+ * gC[g E + e] += c for every visit of state g that cut e, gS[g] +=
+ * c (1 + a(g) lY) for the radius drawn at g, gsum += (c, 1).  The states of
+ * the first three levels (full graph, one and two edges removed) are shared
+ * by many samples: their sums go through LDS (fixed indices: removed edge,
+ * removed pair) and reach global memory once per block. */
+__host__ __device__ constexpr int learn_pairs(int E) { return E * (E - 1) / 2; }
+__host__ __device__ constexpr size_t learn_acc_lds(int E) {
+  return (size_t)(2 + 2 * E + E * E + learn_pairs(E) * (1 + E)) * 8;
+}
+__device__ inline int pair_index(int a, int b) {
+  const int hi = a > b ? a : b, lo = a > b ? b : a;
+  return hi * (hi - 1) / 2 + lo;
+}
+
+template <int E>
+__device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N) {
+  extern __shared__ __align__(16) char lds[];
+  double* sh = reinterpret_cast<double*>(lds);
+  constexpr int NP = learn_pairs(E);
+  double* s_sum = sh;                  /* 2 */
+  double* s_c0 = s_sum + 2;            /* E: cut e0 from the full graph */
+  double* s_s1 = s_c0 + E;             /* E: radius at full \ e0 */
+  double* s_c1 = s_s1 + E;             /* E x E: cut e1 from full \ e0 */
+  double* s_s2 = s_c1 + E * E;         /* NP: radius at full \ {e0, e1} */
+  double* s_c2 = s_s2 + NP;            /* NP x E: cut e2 from full \ {e0, e1} */
+  constexpr int NSH = 2 + 2 * E + E * E + NP * (1 + E);
+  for (int q = threadIdx.x; q < NSH; q += blockDim.x) sh[q] = 0.0;
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) {
+    const double c = L.gbuf[i];
+    atomicAdd(&s_sum[1], 1.0);
+    if (c != 0.0) {
+      const uint64_t path = L.gpath[i];
+      atomicAdd(&s_sum[0], c);
+      uint32_t g = (1u << E) - 1u;
+      int e0 = 0, e1 = 0;
+      for (int j = 0; j < E; j++) {
+        const int e = (int)((path >> (4 * j)) & 15u);
+        const uint32_t cg = g ^ (1u << e);
+        double sv = 0.0;
+        if (j < E - 1) sv = c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[cg]);
+        if (j == 0) {
+          e0 = e;
+          atomicAdd(&s_c0[e], c);
+          if (j < E - 1) atomicAdd(&s_s1[e], sv);
+        } else if (j == 1) {
+          e1 = e;
+          atomicAdd(&s_c1[e0 * E + e], c);
+          if (j < E - 1) atomicAdd(&s_s2[pair_index(e0, e1)], sv);
+        } else if (j == 2) {
+          atomicAdd(&s_c2[pair_index(e0, e1) * E + e], c);
+          if (j < E - 1) unsafeAtomicAdd(&L.gS[cg], sv);
+        } else {
+          unsafeAtomicAdd(&L.gC[(size_t)g * E + e], c);
+          if (j < E - 1) unsafeAtomicAdd(&L.gS[cg], sv);
+        }
+        g = cg;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t full = (1u << E) - 1u;
+  for (int q = threadIdx.x; q < NSH; q += blockDim.x) {
+    const double v = sh[q];
+    if (v == 0.0) continue;
+    if (q < 2) {
+      unsafeAtomicAdd(&L.gsum[q], v);
+    } else if (q < 2 + E) {
+      const int e = q - 2;
+      unsafeAtomicAdd(&L.gC[(size_t)full * E + e], v);
+    } else if (q < 2 + 2 * E) {
+      const int e = q - 2 - E;
+      unsafeAtomicAdd(&L.gS[full ^ (1u << e)], v);
+    } else if (q < 2 + 2 * E + E * E) {
+      const int k = q - 2 - 2 * E, e0 = k / E, e = k % E;
+      unsafeAtomicAdd(&L.gC[(size_t)(full ^ (1u << e0)) * E + e], v);
+    } else {
+      const int k = q - 2 - 2 * E - E * E;
+      int pidx, e = -1;
+      if (k < NP) {
+        pidx = k;
+      } else {
+        pidx = (k - NP) / E;
+        e = (k - NP) % E;
+      }
+      int hi = 1;
+      while ((hi + 1) * hi / 2 <= pidx) hi++;
+      const int lo = pidx - hi * (hi - 1) / 2;
+      const uint32_t gg = full ^ (1u << hi) ^ (1u << lo);
+      if (e < 0) unsafeAtomicAdd(&L.gS[gg], v);
+      else unsafeAtomicAdd(&L.gC[(size_t)gg * E + e], v);
+    }
+  }
+}
+
+/* Forward-KL statistics for E <= FTR_LEARN_LDS_E: every block holds the
+ * whole histogram (gC, gS, gsum) in LDS, walks a strided share of the
+ * recorded samples with LDS atomics and stores its histogram to its own
+ * slice of `part` (no global atomics); learn_reduce_body then sums the
+ * slices in block order, so the statistics are deterministic. */
+#define FTR_LEARN_LDS_E 10
+__host__ __device__ constexpr size_t learn_hist_len(int E) {
+  return ((size_t)E + 1) * ((size_t)1 << E) + 2;
+}
+
+template <int E>
+__device__ inline void learn_hist_body(const LearnLaunch& L, uint64_t N, double* part) {
+  if constexpr (E <= FTR_LEARN_LDS_E) {
+    extern __shared__ __align__(16) char lds[];
+    double* h = reinterpret_cast<double*>(lds);
+    constexpr uint32_t NT = 1u << E;
+    constexpr size_t HL = learn_hist_len(E);
+    double* hC = h;                 /* NT x E */
+    double* hS = h + (size_t)NT * E; /* NT */
+    double* hsum = hS + NT;         /* 2 */
+    for (size_t q = threadIdx.x; q < HL; q += blockDim.x) h[q] = 0.0;
+    __syncthreads();
+    double c_sum = 0.0, n_sum = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+      const double c = L.gbuf[i];
+      n_sum += 1.0;
+      if (c == 0.0) continue;
+      c_sum += c;
+      const uint64_t path = L.gpath[i];
+      uint32_t g = NT - 1u;
+      for (int j = 0; j < E; j++) {
+        const int e = (int)((path >> (4 * j)) & 15u);
+        const uint32_t cg = g ^ (1u << e);
+        atomicAdd(&hC[(size_t)g * E + e], c);
+        if (j < E - 1) atomicAdd(&hS[cg], c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[cg]));
+        g = cg;
+      }
+    }
+    c_sum = wave_sum(c_sum);
+    n_sum = wave_sum(n_sum);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&hsum[0], c_sum);
+      atomicAdd(&hsum[1], n_sum);
+    }
+    __syncthreads();
+    double* out = part + (size_t)blockIdx.x * HL;
+    for (size_t q = threadIdx.x; q < HL; q += blockDim.x) out[q] = h[q];
+  }
+}
+
+/* gC, gS, gsum = sums of the nblk block histograms (one thread per entry). */
+template <int E>
+__device__ inline void learn_reduce_body(const LearnLaunch& L, const double* part, int nblk) {
+  constexpr uint32_t NT = 1u << E;
+  constexpr size_t HL = learn_hist_len(E);
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= HL) return;
+  double sb[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* 8 loads in flight, fixed order */
+  int b = 0;
+  for (; b + 8 <= nblk; b += 8)
+#pragma unroll
+    for (int t = 0; t < 8; t++) sb[t] += part[(size_t)(b + t) * HL + q];
+  for (; b < nblk; b++) sb[0] += part[(size_t)b * HL + q];
+  const double s = ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
+  if (q < (size_t)NT * E) L.gC[q] = s;
+  else if (q < (size_t)NT * E + NT) L.gS[q - (size_t)NT * E] = s;
+  else L.gsum[q - (size_t)NT * E - NT] = s;
+}
+
 /* Learned sampler only (parity): X, the weight p/q and the cut order of
  * samples [first, first+n) of the stream. */
 template <class G, int TMODE>
@@ -950,9 +1075,9 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   const uint64_t i0 = lane * a.S;
   const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
   for (uint64_t i = i0; i < i1; i++) {
-    double X[E], lYs[E], lF, lGv, wgt;
+    double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, TableView<TMODE>, false>(L, rng, tv, X, lF, lGv, wgt, path, lYs);
+    learn_point<G, TableView<TMODE>, false>(L, rng, tv, X, lF, lGv, wgt, path, nullptr, 0);
 #pragma unroll
     for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
     wout[i] = wgt;
@@ -987,6 +1112,17 @@ __device__ inline void batch_body(const BatchItem* items, int nitems) {
   extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_lsm(ftr::LearnLaunch L, double* X,     \
                                                                    double* w, uint64_t* path) {   \
     ftr::learn_sample_body<G, TMODE>(L, X, w, path);                                              \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(256) NAME##_acc(ftr::LearnLaunch L, uint64_t N) {    \
+    ftr::learn_acc_body<G::E>(L, N);                                                              \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(1024) NAME##_hst(ftr::LearnLaunch L, uint64_t N,     \
+                                                                double* part) {                   \
+    ftr::learn_hist_body<G::E>(L, N, part);                                                       \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(256) NAME##_hrd(ftr::LearnLaunch L, const double* part, \
+                                                               int nblk) {                        \
+    ftr::learn_reduce_body<G::E>(L, part, nblk);                                                  \
   }
 
 /* Instantiation macro used by the generated kernel units:

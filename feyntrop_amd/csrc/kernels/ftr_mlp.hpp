@@ -73,6 +73,7 @@ __device__ inline void gemm_tile(const AT* A, size_t sai, size_t sak, const BT* 
   f32x16 acc;
 #pragma unroll
   for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+#pragma unroll 4
   for (int k0 = kbeg; k0 < kend; k0 += 8) {
     bf16x4 a, b;
 #pragma unroll
@@ -125,9 +126,13 @@ extern "C" __global__ void mlp_colsum(const float* dZ, float* db, int rows, int 
   if (j >= cols) return;
   const int i0 = blockIdx.y * chunk;
   const int i1 = i0 + chunk < rows ? i0 + chunk : rows;
-  float s = 0.0f;
-  for (int i = i0; i < i1; i++) s += dZ[(size_t)i * cols + j];
-  unsafeAtomicAdd(&db[j], s);
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}; /* 4 loads in flight */
+  int i = i0;
+  for (; i + 4 <= i1; i += 4)
+#pragma unroll
+    for (int t = 0; t < 4; t++) s[t] += dZ[(size_t)(i + t) * cols + j];
+  for (; i < i1; i++) s[0] += dZ[(size_t)i * cols + j];
+  unsafeAtomicAdd(&db[j], (s[0] + s[1]) + (s[2] + s[3]));
 }
 
 /* Adam on FP32 master parameters, refreshing their bf16 copies. */

@@ -158,7 +158,9 @@ int ftr_precompile(const ftr_input* in, char* err, int errlen);
  * by the exact FP64 ratio p_trop(path)/q(path) keeps the estimator unbiased
  * for any network; training minimises the forward KL divergence between
  * |integrand| x p_trop and q.  Needs E <= 16.  An untrained learner
- * (zero output layer) reproduces the plain sampler sample for sample. */
+ * (zero output layer) reproduces the plain sampler sample for sample.
+ * Network input per state: edge bits, popcount/E, 1, r(g)/E and the scalar
+ * products among the vertices with external momentum (the kinematics). */
 typedef struct ftr_learner ftr_learner;
 #define FTR_LEARN_HIDDEN 128
 #define FTR_LEARN_NPARAMS (128 * 32 + 128 + 2 * (128 * 128 + 128) + 32 * 128 + 32)
@@ -178,12 +180,23 @@ int ftr_learner_backward(ftr_ctx* ctx, ftr_learner* l, const float* dY, float* g
  * the kernel milliseconds. */
 int ftr_learner_train(ftr_ctx* ctx, ftr_learner* l, int steps, uint64_t samples, uint64_t seed, double lr,
                       double* stats);
+/* Training objective: 0 forward KL between |f| p_trop and q (default; the
+ * statistics weight each path by c = |f| p_trop/q), 1 the estimator's
+ * variance E_q[(f p_trop/q)^2] (c = (|f| p_trop/q)^2). */
+int ftr_learner_set_objective(ftr_learner* l, int objective);
 /* Current tables, host arrays: lw, lratio (2^E x E); lZ, lia, lrad, ldiff (2^E). */
 int ftr_learner_tables(ftr_ctx* ctx, ftr_learner* l, double* lw, double* lZ, double* lratio, double* lia,
                        double* lrad, double* ldiff);
 /* Integrate with the learned proposal: moments of R * p_trop/q. */
 int ftr_learner_integrate(ftr_ctx* ctx, ftr_learner* l, uint64_t N, uint64_t seed, const ftr_opts* opts,
                           ftr_moments* out);
+/* Plain sampling as the judge of the network: runs both proposals on
+ * n_probe samples and compares their efficiencies (relative variance of the
+ * eps^0 coefficient x kernel time per sample, lower is better); when the
+ * learned one is not better, ftr_learner_integrate falls back to the plain
+ * sampler (ftr_integrate).  Outputs are optional. */
+int ftr_learner_select(ftr_ctx* ctx, ftr_learner* l, uint64_t n_probe, uint64_t seed, int* use_learned,
+                       double* eff_plain, double* eff_learned);
 /* Proposal samples [first, first+n): X (n*E), weight p/q (n), cut order
  * (n, 4 bits per step, first cut lowest) into device buffers. */
 int ftr_learner_sample(ftr_ctx* ctx, ftr_learner* l, uint64_t first, uint64_t n, uint64_t seed, double* X_dev,

@@ -38,8 +38,19 @@ def bf16(x):
     return (u.astype(np.uint32)).view(np.float32).astype(np.float64)
 
 
-def features(E, r):
-    """Network input per state g: edge bits, popcount/E, 1, r(g)/E."""
+def kinematics(P):
+    """Scalar products among the vertices with external momentum (rows of P
+    with a nonzero entry): upper triangle, row-major, at most 13, scaled by
+    max(1, their largest magnitude)."""
+    P = np.asarray(P, dtype=np.float64)
+    ext = [i for i in range(P.shape[0]) if np.any(P[i] != 0.0)]
+    kin = [P[a, b] for ia, a in enumerate(ext) for b in ext[ia:]][:NIN - MAX_E - 3]
+    ks = max([1.0] + [abs(v) for v in kin])
+    return np.array(kin) / ks
+
+
+def features(E, r, P=None):
+    """Network input per state g: edge bits, popcount/E, 1, r(g)/E, kinematics."""
     n = 1 << E
     M = max(32, n)
     X = np.zeros((M, NIN))
@@ -49,6 +60,9 @@ def features(E, r):
     X[:n, MAX_E] = np.array([bin(int(v)).count("1") for v in g], dtype=np.float64) / E
     X[:n, MAX_E + 1] = 1.0
     X[:n, MAX_E + 2] = (np.asarray(r, dtype=np.float64) / E).astype(np.float32)
+    if P is not None:
+        kin = kinematics(P)
+        X[:n, MAX_E + 3:MAX_E + 3 + len(kin)] = kin
     return X.astype(np.float32).astype(np.float64)
 
 

@@ -63,7 +63,7 @@ def test_network_forward_and_backward(engine, name):
     lr.set_params(par)
     np.testing.assert_array_equal(lr.params(), par)
     Jr, r, _, _ = p.jtable()
-    X = LR.features(p.E, r)
+    X = LR.features(p.E, r, prob["scalarproducts"])
     Y = lr.forward().astype(np.float64)
     Yr, cache = LR.forward(par.astype(np.float64), X, emulate=True)
     n = 1 << p.E
@@ -150,6 +150,29 @@ def test_training_unbiased_and_lowers_variance(engine, name):
             assert abs(a - b) <= 3.0 * np.hypot(da, db) + 1e-300, (k, q, a, da, b, db)
     gain = _rel2(p, mp) / _rel2(p, ml)
     assert gain > 1.5, gain
+
+
+def test_select_keeps_better_proposal(engine):
+    """Plain sampling judges the network: a trained learner on the triple
+    ladder is kept; a learner with a deliberately bad tilt (random output
+    layer) loses to the plain sampler, and integrate() then returns the plain
+    sampler's moments exactly."""
+    prob, p = _setup(engine, "triple_ladder")
+    good = feyntrop_amd.Learner(engine, p, init_seed=1)
+    good.train(steps=150, samples=1 << 18, seed=100, lr=3e-3)
+    use, ep, el = good.select(n_probe=2_000_000)
+    assert use and el < ep
+    bad = feyntrop_amd.Learner(engine, p, init_seed=1)
+    par = bad.params()
+    d = LR.unpack(par.astype(np.float64))
+    rng = np.random.default_rng(0)
+    d["W4"] = rng.normal(0, 3.0, d["W4"].shape)
+    bad.set_params(LR.pack(d).astype(np.float32))
+    use, ep, el = bad.select(n_probe=2_000_000)
+    assert not use and el >= ep
+    m0 = engine.integrate(p, 1_000_000, 5)
+    m1 = bad.integrate(1_000_000, 5)
+    assert m1.s1[0][0] == m0.s1[0][0] and m1.s2[0][0] == m0.s2[0][0]
 
 
 def test_learned_wheel_period_6zeta3(engine):
