@@ -29,6 +29,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
+# HBM traffic of one ftrk_int launch (N = 1e8) from two rocprofv3 --pmc runs of
+# this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean of the
+# timed launches), each scaled by what the same counter read for a 256 MiB
+# device-to-device copy in the same run (FETCH 131086.8 KiB, WRITE 262144 KiB
+# for 262144 KiB moved).  Summary and command under profiles/.
+PMC_TRAFFIC_BYTES = {
+    "double_box": (457.458 * (262144 / 131086.8125) + 24.0 * (262144 / 262144)) * 1024,
+}
+
 
 def flops_per_sample(info):
     """Algorithmic FP64 flops of one sample (+ - x /, FMA = 2; transcendentals
@@ -160,6 +169,8 @@ def run_learned(eng, name="triple_ladder", steps=400, batch=1 << 18, lr=3e-3):
     rel_l, Il, rl = _rel_err(p, ml)
     t13_p = (rel_p / 1e-3) ** 2 * tp
     t13_l = (rel_l / 1e-3) ** 2 * tl + t_train
+    # equal wall-clock: the plain sampler given the learned run's time to 1e-3
+    rel_p_eq = 1e-3 * np.sqrt(t13_p / t13_l)
     z = [abs(rl[k, q, 0] - rp[k, q, 0]) / max(np.hypot(rl[k, q, 1], rp[k, q, 1]), 1e-300)
          for k in range(p.K) for q in range(2)]
     fps = flops_per_sample(p.info)
@@ -176,9 +187,27 @@ def run_learned(eng, name="triple_ladder", steps=400, batch=1 << 18, lr=3e-3):
                         "roofline_frac": fps * N / (ml.kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
             "variance_gain": (rel_p ** 2) / (rel_l ** 2),
             "speedup_time_to_1e-3": t13_p / t13_l,
-            "rel_err_at_equal_wallclock": {"plain": rel_p, "learned": rel_l * np.sqrt((tl + t_train) / tp)},
+            "rel_err_at_equal_wallclock": {"wall_s": t13_l, "plain": rel_p_eq, "learned": 1e-3},
             "max_sigma_plain_vs_learned": float(max(z)),
             "eps0": {"plain": [Ip.real, Ip.imag], "learned": [Il.real, Il.imag]}}
+
+
+def calibration_copy(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel
+    reads nbytes and writes nbytes): the reference for scaling the HBM
+    traffic counters of the sampling kernel."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"calibration copy: {nbytes >> 20} MiB read + {nbytes >> 20} MiB written", file=sys.stderr)
 
 
 def main():
@@ -193,6 +222,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (the "
+                         "reference kernel for scaling HBM traffic counters)")
     ap.add_argument("--scan-points", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -243,7 +275,8 @@ def main():
         "rel_err_eps0_at_N": rel_at_N,
         "eps0": [I0.real, I0.imag],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": PMC_TRAFFIC_BYTES.get(args.workload) if N == 10**8 else None,
                      "kernel": "ftrk_int", "flops_per_sample": fps, "kernel_ms_mean": mean_kms},
         "learned": None,
     }
@@ -267,6 +300,8 @@ def main():
         ns = args.cpu_samples or 2.5e6 * threads
         line["cpu_baseline"] = cpu_baseline(prob, ns, threads)
     print(json.dumps(line), flush=True)
+    if args.calib_copy_mib:
+        calibration_copy(args.calib_copy_mib << 20)
 
 
 if __name__ == "__main__":
