@@ -228,6 +228,30 @@ def test_cli_readme_wheel(golden_fixtures):
     assert "(Effective) kinematic regime: Euclidean (generic)." in out.stderr
 
 
+def test_cli_batch_mode_equals_single_runs():
+    """{"batch": [...]}: one result per problem, each the single-problem run
+    of the binary: the same samples, so IGtr identical and every value and
+    error equal up to the summation order of the moments (1e-10 / 1e-8)."""
+    import feyntrop_amd
+    probs = [W.wheel_k4(N=200000, seed=3), W.double_box(N=300000, seed=5), W.fixture_2L_3pt_tutorial(N=100000)]
+    out = subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps({"batch": probs}), capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout)
+    assert list(d.keys()) == ["batch"] and len(d["batch"]) == 3
+    for prob, res in zip(probs, d["batch"]):
+        one = subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps(prob), capture_output=True, text=True,
+                             timeout=300)
+        assert one.returncode == 0, one.stderr
+        single = json.loads(one.stdout)
+        assert list(res.keys()) == list(single.keys())
+        assert res["IGtr"] == single["IGtr"]
+        a, b = np.array(res["integral"]), np.array(single["integral"])
+        assert a.shape == b.shape
+        np.testing.assert_allclose(a[..., 0], b[..., 0], rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(a[..., 1], b[..., 1], rtol=1e-8, atol=1e-14)
+
+
 def test_batch_equals_individual(engine):
     """The batched scan draws the same samples as one call per problem: the
     same counts, sums equal up to summation order."""

@@ -90,6 +90,18 @@ def test_cli_rejects_bad_input(label, mut, msg):
     assert msg in r.stderr
 
 
+def test_cli_batch_rejects_bad_item():
+    """Batch mode names the failing item and keeps the item's own message."""
+    good = W.wheel_k4(N=1000)
+    bad = W.wheel_k4(N=1000)
+    bad["N"] = "many"
+    r = _cli({"batch": [good, bad]})
+    assert r.returncode != 0 and r.stdout == ""
+    assert "batch item 1: input N must be a number" in r.stderr
+    r = _cli({"batch": []})
+    assert r.returncode != 0 and "non-empty list" in r.stderr
+
+
 def test_cli_rejects_malformed_json():
     r = _cli("{\"N\": 10, ")
     assert r.returncode != 0 and r.stdout == ""
