@@ -96,6 +96,7 @@ def lib():
     L.ftr_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
     L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
+    L.ftr_set_jtable_mode.argtypes = [vp, C.c_int]
     L.ftr_precompile_learned.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_learner_create.argtypes = [vp, vp, C.c_uint64, C.POINTER(vp)]
     L.ftr_learner_free.argtypes = [vp]
@@ -279,6 +280,12 @@ class Engine:
 
     def set_variant(self, p, table_mode, block):
         rc = lib().ftr_set_variant(self.h, p.h, int(table_mode), int(block))
+        if rc:
+            raise self._err(rc)
+
+    def set_jtable_mode(self, mode):
+        """J-table construction: 0 auto, 1 host, 2 device."""
+        rc = lib().ftr_set_jtable_mode(self.h, int(mode))
         if rc:
             raise self._err(rc)
 

@@ -1,6 +1,7 @@
 """Build step: embed the device headers into the runtime library as C strings:
 ftr_abi.hpp + ftr_device.hpp (the source hipRTC specialises per graph) and
-ftr_mlp.hpp (the learned proposal's network, compiled once)."""
+ftr_mlp.hpp (the learned proposal's network) and ftr_jtable.hpp (the
+J-table), each compiled once."""
 import os
 import sys
 
@@ -13,6 +14,8 @@ delim = "FTRSRC"
 assert (")" + delim) not in src
 mlp = open(os.path.join(here, "kernels", "ftr_mlp.hpp")).read().replace("#pragma once", "")
 assert (")" + delim) not in mlp
+jt = open(os.path.join(here, "kernels", "ftr_jtable.hpp")).read().replace("#pragma once", "")
+assert (")" + delim) not in jt
 out = sys.argv[1]
 
 
@@ -29,4 +32,5 @@ with open(out + ".tmp", "w") as f:
     f.write("/* generated by embed_source.py -- do not edit */\n")
     literal(f, "ftr_embedded_device_source", src)
     literal(f, "ftr_embedded_mlp_source", mlp)
+    literal(f, "ftr_embedded_jtable_source", jt)
 os.replace(out + ".tmp", out)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,26 @@ struct Graph {
   std::vector<std::vector<std::pair<int, int>>> adj;
 };
 
+/* Inputs of the J-table construction shared by the host and device
+ * implementations (tropical_sampling.hpp:284-458). */
+struct JtSetup {
+  int n_tests = 0;
+  const double* Xs = nullptr; /* n_tests x E random points in [1, 3) */
+  const double* Xi = nullptr; /* their inverses */
+  double pmsqr_ref = 0, W = 0, W_ref = 0;
+  bool phi = false, noneucl = false;
+};
+/* Flags of the J-table pass; div_level >= 0 names the first subset size
+ * with a vanishing effective degree of divergence and div_subset its first
+ * subset in increasing numeric (Gosper) order. */
+struct JtResult {
+  bool generic = true, pseudo = true, gp = true;
+  int div_level = -1;
+  uint64_t div_subset = 0;
+};
+struct Prep;
+void host_jtable(Prep& pr, const JtSetup& js, JtResult& res);
+
 /* Host-side preprocessing results for one problem. */
 struct Prep {
   Graph g;
@@ -38,6 +59,9 @@ struct Prep {
   std::vector<uint8_t> mm, Lt;  /* 2^E */
   std::vector<std::string> log; /* regime / GP log lines, in reference order */
   double seconds = 0;
+  bool jtable_on_device = false; /* set when jtable_fn built the table */
+  /* optional replacement of host_jtable (the device J-table) */
+  std::function<void(Prep&, const JtSetup&, JtResult&)> jtable_fn;
 };
 
 void build_graph(Graph& g, int E, const int32_t* edges, const double* nu);

@@ -15,6 +15,7 @@
 
 extern const char* ftr_embedded_device_source;
 extern const char* ftr_embedded_mlp_source;
+extern const char* ftr_embedded_jtable_source;
 
 using ftrh::Error;
 
@@ -44,6 +45,7 @@ struct ftr_ctx {
   size_t batch_bytes = 0;
   double* h_pinned = nullptr; /* pinned copy-back buffer of the batch graph */
   size_t pinned_bytes = 0;
+  int jtable_mode = 0;        /* 0 auto, 1 host, 2 device (ftr_set_jtable_mode) */
 };
 
 struct ftr_problem {
@@ -90,4 +92,8 @@ void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
 Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit);
 size_t variant_lds(const Variant& v, int E);
 int fail(ftr_ctx* c, int code, const std::string& msg);
+/* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */
+void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& res);
+/* smallest E for which ftr_prepare builds the J-table on the device in auto mode */
+constexpr int FTR_DEVICE_JTABLE_MIN_E = 13;
 extern thread_local std::string g_host_err;

@@ -129,6 +129,10 @@ static double ldlt_trace_solve(Small& A, const Small& B) {
   double y[32];
   const double tol = std::numeric_limits<double>::min();
   for (int c = 0; c < n; c++) {
+    /* a zero column of B adds exactly zero */
+    bool nz = false;
+    for (int i = 0; i < n; i++) nz = nz || (B(i, c) != 0.0);
+    if (!nz) continue;
     for (int i = 0; i < n; i++) y[i] = B(perm[i], c);
     for (int i = 0; i < n; i++)
       for (int q = 0; q < i; q++) y[i] -= A(i, q) * y[q];
@@ -205,107 +209,24 @@ static std::string matrix_str(const std::vector<double>& P, int V) {
   return s.str();
 }
 
-void preprocess(Prep& pr) {
-  const auto t0 = std::chrono::steady_clock::now();
+/* The J-table on the host (tropical_sampling.hpp:284-458): subsets level by
+ * level (size n), each level in Gosper order, OpenMP over a level. */
+void host_jtable(Prep& pr, const JtSetup& js, JtResult& res) {
   const Graph& g = pr.g;
   const int V = g.V, E = g.E;
   const std::vector<double>& P = pr.P;
-  const double D = pr.D;
-  if ((int)pr.m2.size() != E) throw Error(-1, "input masses_sqr must be an array of size E");
-  /* feyntrop.cpp:98-112 */
-  {
-    std::vector<double> Pt((size_t)V * V);
-    for (int i = 0; i < V; i++)
-      for (int j = 0; j < V; j++) Pt[(size_t)j * V + i] = P[(size_t)i * V + j];
-    if (!is_approx(P, Pt))
-      throw Error(-2, "integrate_graph: The matrix \n" + matrix_str(P, V) +
-                          "\nof invariants was given as input and it is not symmetric or not a VxV matrix. One row and "
-                          "column must be specified for each vertex - representing the momentum flowing into this vertex.");
-    for (int j = 0; j < V; j++) {
-      double s = 0;
-      for (int i = 0; i < V; i++) s += P[(size_t)i * V + j];
-      if (!(std::fabs(s) <= 1e-12))
-        throw Error(-2, "Scalar product (P) matrix has to fulfill momentum conservation. That means all row and column "
-                        "sums need to be 0. The matrix \n" + matrix_str(P, V) + "\ndoes not fulfill this requirement.");
-    }
-  }
-  /* feyntrop.cpp:128-133: Euclidean iff P == -|P| */
-  {
-    std::vector<double> w, Q;
-    sym_eig(V, P, w, Q);
-    std::vector<double> Pabs((size_t)V * V, 0.0);
-    for (int i = 0; i < V; i++)
-      for (int j = 0; j < V; j++) {
-        double s = 0;
-        for (int k = 0; k < V; k++) s += Q[(size_t)i * V + k] * std::fabs(w[k]) * Q[(size_t)j * V + k];
-        Pabs[(size_t)i * V + j] = -s;
-      }
-    bool zero = true;
-    for (double v : P)
-      if (!(std::fabs(v) <= 1e-12)) zero = false;
-    pr.euclidean = zero || is_approx(P, Pabs);
-  }
-  pr.ext.clear();
-  for (int i = 0; i < V; i++) {
-    bool nz = false;
-    for (int j = 0; j < V; j++)
-      if (P[(size_t)i * V + j] != 0.0) nz = true;
-    if (nz) pr.ext.push_back(i);
-  }
-  /* tropical_sampling.hpp:284-458 */
-  if (E > 63) {
-    std::stringstream s;
-    s << "E = " << E << " this graph is too large for this implementation!";
-    throw Error(-2, s.str());
-  }
-  if (V > 32) throw Error(-5, "Error: stacksize for BFS stack too small. Compile with larger MAX_BFS_STACKSIZE.");
-  if (E > 30) throw Error(-5, "E > 30: the 2^E subgraph table does not fit this engine");
-  std::vector<int> cm(V);
-  const uint64_t full = (UINT64_C(1) << E) - 1;
-  const int C = components(g, full, cm.data());
-  const int L = E - V + C;
-  if (C != 1) throw Error(-2, "Error: the graph " + graph_str(g) + " is not connected.");
-  const double W = omega(g, D, L, full), W_ref = omega(g, -D, L, full);
-  pr.L = L;
-  pr.W = W;
-  pr.W_ref = W_ref;
-  double pgp0 = 0;
-  for (int i = 0; i < V - 1; i++)
-    for (int j = 0; j < V - 1; j++) pgp0 += P[(size_t)i * V + j] * P[(size_t)i * V + j];
-  double m2n = 0;
-  for (double v : pr.m2) m2n += v * v;
-  const double pmsqr_ref = std::sqrt(pgp0) + std::sqrt(m2n);
-  const bool bPhiTr = (pr.K > 1) || std::fabs(W / W_ref) > 1e-12;
-  if (bPhiTr && std::fabs(pmsqr_ref) < 1e-12) {
-    std::stringstream s;
-    s << "The graph " << graph_str(g) << " has superficial degree of divergence " << W
-      << " and/or we need higher orders in epsilon. So, kinematics are necessary. The incoming momenta/masses appear "
-         "to be too small to get an (numerically effective) non-zero F/Phi polynomial.";
-    throw Error(-2, s.str());
-  }
-  const bool bNonEuclidean = !pr.euclidean;
-  /* random test points X in [1, 3) for the mass-momentum-spanning test */
-  const int n_tests = 100;
-  std::vector<double> Xs((size_t)n_tests * E), Xi((size_t)n_tests * E);
-  {
-    uint64_t st[4];
-    xo_seed(0x6d6d7465737473ULL, st);
-    for (size_t q = 0; q < Xs.size(); q++) {
-      const uint64_t res = st[0] + st[3], t = st[1] << 17;
-      st[2] ^= st[0]; st[3] ^= st[1]; st[1] ^= st[2]; st[0] ^= st[3]; st[2] ^= t;
-      st[3] = (st[3] << 45) | (st[3] >> 19);
-      Xs[q] = 1.0 + 2.0 * ((double)(res >> 11) * (1.0 / 9007199254740992.0));
-      Xi[q] = 1.0 / Xs[q];
-    }
-  }
+  const double D = pr.D, W = js.W, W_ref = js.W_ref, pmsqr_ref = js.pmsqr_ref;
+  const bool bPhiTr = js.phi, bNonEuclidean = js.noneucl;
+  const int n_tests = js.n_tests;
+  const double* Xs = js.Xs;
+  const double* Xi = js.Xi;
   const size_t NT = (size_t)1 << E;
   pr.Jr.assign(NT, 0.0);
   pr.r.assign(NT, 0.0);
   pr.mm.assign(NT, 0);
   pr.Lt.assign(NT, 0);
   bool bGeneric = true, bPseudo = true, bGP = true;
-  std::string div_err;
-  for (int n = 0; n <= E && div_err.empty(); n++) {
+  for (int n = 0; n <= E && res.div_level < 0; n++) {
     /* subsets of size n in Gosper order */
     std::vector<uint64_t> subs;
     uint64_t sub = (n == 0) ? 0 : ((UINT64_C(1) << n) - 1);
@@ -434,17 +355,133 @@ void preprocess(Prep& pr) {
     bPseudo = bPseudo && lvPseudo;
     bGP = bGP && lvGP;
     if (first_div < ns) {
-      const uint64_t s = subs[first_div];
-      std::stringstream m;
-      m << "Error: the graph " << graph_str(g) << " has a UV or IR subdivergence. \nE.g. the subgraph with edge numbers { ";
-      for (int j = 0; j < E; j++)
-        if ((s >> j) & 1) m << j << " ";
-      m << "} or, explicitly, with the edges { ";
-      for (int j = 0; j < E; j++)
-        if ((s >> j) & 1) m << "[" << g.hi[j] << "," << g.lo[j] << "] ";
-      m << "} has an (effective) superficial degree of divergence of " << pr.r[s] << ".";
-      div_err = m.str();
+      res.div_level = n;
+      res.div_subset = subs[first_div];
     }
+  }
+  res.generic = bGeneric;
+  res.pseudo = bPseudo;
+  res.gp = bGP;
+}
+
+void preprocess(Prep& pr) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const Graph& g = pr.g;
+  const int V = g.V, E = g.E;
+  const std::vector<double>& P = pr.P;
+  const double D = pr.D;
+  if ((int)pr.m2.size() != E) throw Error(-1, "input masses_sqr must be an array of size E");
+  /* feyntrop.cpp:98-112 */
+  {
+    std::vector<double> Pt((size_t)V * V);
+    for (int i = 0; i < V; i++)
+      for (int j = 0; j < V; j++) Pt[(size_t)j * V + i] = P[(size_t)i * V + j];
+    if (!is_approx(P, Pt))
+      throw Error(-2, "integrate_graph: The matrix \n" + matrix_str(P, V) +
+                          "\nof invariants was given as input and it is not symmetric or not a VxV matrix. One row and "
+                          "column must be specified for each vertex - representing the momentum flowing into this vertex.");
+    for (int j = 0; j < V; j++) {
+      double s = 0;
+      for (int i = 0; i < V; i++) s += P[(size_t)i * V + j];
+      if (!(std::fabs(s) <= 1e-12))
+        throw Error(-2, "Scalar product (P) matrix has to fulfill momentum conservation. That means all row and column "
+                        "sums need to be 0. The matrix \n" + matrix_str(P, V) + "\ndoes not fulfill this requirement.");
+    }
+  }
+  /* feyntrop.cpp:128-133: Euclidean iff P == -|P| */
+  {
+    std::vector<double> w, Q;
+    sym_eig(V, P, w, Q);
+    std::vector<double> Pabs((size_t)V * V, 0.0);
+    for (int i = 0; i < V; i++)
+      for (int j = 0; j < V; j++) {
+        double s = 0;
+        for (int k = 0; k < V; k++) s += Q[(size_t)i * V + k] * std::fabs(w[k]) * Q[(size_t)j * V + k];
+        Pabs[(size_t)i * V + j] = -s;
+      }
+    bool zero = true;
+    for (double v : P)
+      if (!(std::fabs(v) <= 1e-12)) zero = false;
+    pr.euclidean = zero || is_approx(P, Pabs);
+  }
+  pr.ext.clear();
+  for (int i = 0; i < V; i++) {
+    bool nz = false;
+    for (int j = 0; j < V; j++)
+      if (P[(size_t)i * V + j] != 0.0) nz = true;
+    if (nz) pr.ext.push_back(i);
+  }
+  /* tropical_sampling.hpp:284-458 */
+  if (E > 63) {
+    std::stringstream s;
+    s << "E = " << E << " this graph is too large for this implementation!";
+    throw Error(-2, s.str());
+  }
+  if (V > 32) throw Error(-5, "Error: stacksize for BFS stack too small. Compile with larger MAX_BFS_STACKSIZE.");
+  if (E > 30) throw Error(-5, "E > 30: the 2^E subgraph table does not fit this engine");
+  std::vector<int> cm(V);
+  const uint64_t full = (UINT64_C(1) << E) - 1;
+  const int C = components(g, full, cm.data());
+  const int L = E - V + C;
+  if (C != 1) throw Error(-2, "Error: the graph " + graph_str(g) + " is not connected.");
+  const double W = omega(g, D, L, full), W_ref = omega(g, -D, L, full);
+  pr.L = L;
+  pr.W = W;
+  pr.W_ref = W_ref;
+  double pgp0 = 0;
+  for (int i = 0; i < V - 1; i++)
+    for (int j = 0; j < V - 1; j++) pgp0 += P[(size_t)i * V + j] * P[(size_t)i * V + j];
+  double m2n = 0;
+  for (double v : pr.m2) m2n += v * v;
+  const double pmsqr_ref = std::sqrt(pgp0) + std::sqrt(m2n);
+  const bool bPhiTr = (pr.K > 1) || std::fabs(W / W_ref) > 1e-12;
+  if (bPhiTr && std::fabs(pmsqr_ref) < 1e-12) {
+    std::stringstream s;
+    s << "The graph " << graph_str(g) << " has superficial degree of divergence " << W
+      << " and/or we need higher orders in epsilon. So, kinematics are necessary. The incoming momenta/masses appear "
+         "to be too small to get an (numerically effective) non-zero F/Phi polynomial.";
+    throw Error(-2, s.str());
+  }
+  const bool bNonEuclidean = !pr.euclidean;
+  /* random test points X in [1, 3) for the mass-momentum-spanning test */
+  const int n_tests = 100;
+  std::vector<double> Xs((size_t)n_tests * E), Xi((size_t)n_tests * E);
+  {
+    uint64_t st[4];
+    xo_seed(0x6d6d7465737473ULL, st);
+    for (size_t q = 0; q < Xs.size(); q++) {
+      const uint64_t res = st[0] + st[3], t = st[1] << 17;
+      st[2] ^= st[0]; st[3] ^= st[1]; st[1] ^= st[2]; st[0] ^= st[3]; st[2] ^= t;
+      st[3] = (st[3] << 45) | (st[3] >> 19);
+      Xs[q] = 1.0 + 2.0 * ((double)(res >> 11) * (1.0 / 9007199254740992.0));
+      Xi[q] = 1.0 / Xs[q];
+    }
+  }
+  JtSetup js;
+  js.n_tests = n_tests;
+  js.Xs = Xs.data();
+  js.Xi = Xi.data();
+  js.pmsqr_ref = pmsqr_ref;
+  js.W = W;
+  js.W_ref = W_ref;
+  js.phi = bPhiTr;
+  js.noneucl = bNonEuclidean;
+  JtResult jres;
+  if (pr.jtable_fn) pr.jtable_fn(pr, js, jres);
+  else host_jtable(pr, js, jres);
+  const bool bGeneric = jres.generic, bPseudo = jres.pseudo, bGP = jres.gp;
+  std::string div_err;
+  if (jres.div_level >= 0) {
+    const uint64_t s = jres.div_subset;
+    std::stringstream m;
+    m << "Error: the graph " << graph_str(g) << " has a UV or IR subdivergence. \nE.g. the subgraph with edge numbers { ";
+    for (int j = 0; j < E; j++)
+      if ((s >> j) & 1) m << j << " ";
+    m << "} or, explicitly, with the edges { ";
+    for (int j = 0; j < E; j++)
+      if ((s >> j) & 1) m << "[" << g.hi[j] << "," << g.lo[j] << "] ";
+    m << "} has an (effective) superficial degree of divergence of " << pr.r[s] << ".";
+    div_err = m.str();
   }
   if (!div_err.empty()) throw Error(-2, div_err);
   pr.IGtr = pr.Jr[full];

@@ -384,6 +384,10 @@ int ftr_create(int device, ftr_ctx** out) {
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    if (const char* m = getenv("FEYNTROP_AMD_JTABLE")) {
+      if (!strcmp(m, "host")) c->jtable_mode = 1;
+      else if (!strcmp(m, "device")) c->jtable_mode = 2;
+    }
   } catch (Error& e) {
     fprintf(stderr, "feyntrop_amd: %s\n", e.what());
     delete c;
@@ -418,6 +422,7 @@ int ftr_precompile(const ftr_input* in, char* err, int errlen) {
   try {
     ftrh::Prep pr;
     prep_from_input(pr, in);
+    code_object(ftr_embedded_jtable_source, nullptr); /* the device J-table module */
     /* both LDS limits a gfx950 part may report */
     for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
       const Variant v = choose_variant(pr, lim);
@@ -442,6 +447,14 @@ int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
   p->device = c ? c->device : 0;
   try {
     const auto t0 = std::chrono::steady_clock::now();
+    if (c && c->jtable_mode != 1) {
+      /* the J-table on the device: always (mode 2) or from E = 13 on (auto) */
+      const int mode = c->jtable_mode;
+      p->prep.jtable_fn = [c, mode](ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& r) {
+        if (mode == 2 || pr.g.E >= FTR_DEVICE_JTABLE_MIN_E) device_jtable(c, pr, js, r);
+        else ftrh::host_jtable(pr, js, r);
+      };
+    }
     prep_from_input(p->prep, in);
     if (!c) {
       /* host-only preparation: validation, regime, J-table */
@@ -918,6 +931,12 @@ int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block) {
 }
 
 void ftr_free_string(char* s) { free(s); }
+
+int ftr_set_jtable_mode(ftr_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 2) return FTR_ERR_INVALID;
+  c->jtable_mode = mode;
+  return FTR_OK;
+}
 
 }  // extern "C"
 

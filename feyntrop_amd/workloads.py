@@ -191,6 +191,20 @@ FIXTURES = {
     "2L_5pt_box_pent": fixture_2L_5pt_box_pent,
 }
 
+def ladder_4pt(rungs=6, N=10**7, seed=0, D=4, mass=1.0):
+    """Planar 4-point ladder with `rungs` loops (V = 2 (rungs + 1), E =
+    3 rungs + 1), Euclidean kinematics s = -3, t = -2 on the corner
+    vertices, all masses equal: a large J-table (2^E entries)."""
+    top = list(range(rungs + 1))
+    bot = [rungs + 1 + i for i in range(rungs + 1)]
+    pairs = [(top[i], bot[i]) for i in range(rungs + 1)]
+    pairs += [(top[i], top[i + 1]) for i in range(rungs)] + [(bot[i], bot[i + 1]) for i in range(rungs)]
+    V = 2 * (rungs + 1)
+    P = _four_point(V, (top[0], bot[0], top[-1], bot[-1]), -3.0, -2.0, -1.0)
+    return {"graph": _edges(pairs), "dimension": D, "scalarproducts": P, "masses_sqr": [mass] * len(pairs),
+            "num_eps_terms": 1, "lambda": 0.0, "N": N, "seed": seed}
+
+
 CONFIGS = {
     "wheel_k4": wheel_k4,
     "readme_wheel": readme_wheel,

@@ -143,6 +143,13 @@ int ftr_autotune(ftr_ctx* ctx, ftr_problem* p, uint64_t n_probe, int* table_mode
 int ftr_set_variant(ftr_ctx* ctx, ftr_problem* p, int table_mode, int block);
 int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block);
 
+/* Where ftr_prepare builds the 2^E-entry J-table (tropical_sampling.hpp:
+ * 284-458): 0 auto (the device from E = 13 on, the host below), 1 host
+ * (OpenMP), 2 device (one launch per subset size, one thread per subset).
+ * The tables and flags are bitwise the same either way.  The environment
+ * variable FEYNTROP_AMD_JTABLE=host|device sets the default of new contexts. */
+int ftr_set_jtable_mode(ftr_ctx* ctx, int mode);
+
 /* Kernel code-object cache directory (default: the package's kcache/). */
 int ftr_set_cache_dir(const char* dir);
 
