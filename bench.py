@@ -115,6 +115,31 @@ def run_scan(eng, n_points, seed=0):
             "dropped": int(sum(m.dropped for m in outs))}
 
 
+def run_large(eng, name="ladder7", steps=2):
+    """A 22-edge planar ladder: the 2^22-entry J-table built on the device
+    (timed against the host pass), then plain sampling with the tables read
+    from L2/HBM."""
+    from feyntrop_amd import workloads as W
+    prob = W.LARGE[name]()
+    out = {}
+    for mode, tag in ((2, "device"), (1, "host")):
+        eng.set_jtable_mode(mode)
+        t0 = time.perf_counter()
+        p = eng.prepare(prob)
+        out[f"prepare_s_{tag}_jtable"] = time.perf_counter() - t0
+    eng.set_jtable_mode(0)
+    N = int(prob["N"])
+    eng.integrate(p, 1_000_000, 1)
+    t0 = time.perf_counter()
+    ms = [eng.integrate(p, N, 100 + i).kernel_ms for i in range(steps)]
+    wall = time.perf_counter() - t0
+    fps = flops_per_sample(p.info)
+    out.update({"workload": f"{name}: E={p.E} V={p.V} K={p.K}, 2^{p.E} J-table entries",
+                "value": N * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
+                "roofline_frac": fps * N / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS})
+    return out
+
+
 def run_tuned(eng, name, steps=2):
     """A named workload with the autotuner's kernel variant."""
     from feyntrop_amd import workloads as W
@@ -289,7 +314,8 @@ def main():
         extras = {}
         for key, fn in (("scan", lambda: run_scan(eng, args.scan_points)),
                         ("cubic5", lambda: run_tuned(eng, "cubic5")),
-                        ("triple_ladder", lambda: run_tuned(eng, "triple_ladder"))):
+                        ("triple_ladder", lambda: run_tuned(eng, "triple_ladder")),
+                        ("large_graph", lambda: run_large(eng))):
             try:
                 extras[key] = fn()
             except Exception as e:  # an extra must not take the main line down

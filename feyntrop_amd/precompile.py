@@ -14,6 +14,7 @@ from feyntrop_amd import host_path, precompile, precompile_learned, workloads
 
 def problems():
     probs = dict(workloads.all_problems())
+    probs["ladder7"] = workloads.LARGE["ladder7"]   # the large-graph bench line
     for k, v in workloads.scan_problems(host_path).items():
         probs[k] = v
     return probs

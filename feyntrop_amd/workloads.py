@@ -214,6 +214,14 @@ CONFIGS = {
 }
 
 
+# large J-tables: device-built 2^E tables, sampler tables read from L2/HBM
+LARGE = {
+    "ladder5": lambda N=10**7, seed=0: ladder_4pt(5, N, seed),
+    "ladder6": lambda N=10**7, seed=0: ladder_4pt(6, N, seed),
+    "ladder7": lambda N=10**8, seed=0: ladder_4pt(7, N, seed),
+}
+
+
 def all_problems():
     d = dict(CONFIGS)
     d.update(FIXTURES)

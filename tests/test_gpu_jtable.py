@@ -73,3 +73,25 @@ def test_device_jtable_subdivergence_message(engine):
     _, ed = _prep(engine, prob, 2)
     assert eh is not None and "subdivergence" in eh
     assert eh == ed
+
+
+def test_large_graph_estimate_equals_oracle(engine, oracle):
+    """16-edge ladder (J-table built on the device, tables read from L2/HBM
+    by the sampler): integrate(N, seed) draws the oracle's one-thread samples
+    (sample count exact, means within 1e-6 sigma, second moments 1e-9)."""
+    prob = W.ladder_4pt(5, N=200_000, seed=3)
+    engine.set_jtable_mode(2)
+    try:
+        p = engine.prepare(prob)
+    finally:
+        engine.set_jtable_mode(0)
+    op = oracle.problem(prob)
+    ref, dropped = op.estimate(prob["N"], prob["seed"], 1, 1)
+    m = engine.integrate(p, prob["N"], prob["seed"])
+    n, s1, s2 = m.arrays()
+    assert m.dropped == dropped
+    rn, rs1, rs2 = ref[0, 0]
+    assert n[0, 0] == rn
+    sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+    assert abs(s1[0, 0] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn)
+    assert abs(s2[0, 0] - rs2) <= 1e-9 * abs(rs2)
