@@ -266,6 +266,14 @@ def main():
     N = int(prob["N"])
     eng = feyntrop_amd.Engine(0)
     p = eng.prepare(prob)
+    # kernel choice (preparation, untimed): the generic kernel or the one
+    # specialised to this kinematic point, whichever is faster on a probe
+    probe = {}
+    for spec in (False, True):
+        eng.set_specialized(p, spec)
+        eng.integrate(p, min(N, 10**7), 7)
+        probe[spec] = eng.integrate(p, min(N, 10**7), 8).kernel_ms
+    eng.set_specialized(p, probe[True] < probe[False])
     for i in range(args.warmup):
         eng.integrate(p, N, i + 1)
     kms, res, seeds = [], [], []
@@ -295,7 +303,8 @@ def main():
         "dtype": "f64", "data": "synthetic (kinematics from seed 0)",
         "config": {"workload": f"{args.workload}: E={p.E} V={p.V} K={p.K} path="
                                f"{['real', 'real-psi', 'deformed-complex'][p.info.path]}",
-                   "N_per_step": N, "lambda": prob["lambda"], "dimension": prob["dimension"]},
+                   "N_per_step": N, "lambda": prob["lambda"], "dimension": prob["dimension"],
+                   "kernel": "specialised to the kinematic point" if eng.specialized(p) else "generic"},
         "time_to_1e-3_s": t_1e3,
         "rel_err_eps0_at_N": rel_at_N,
         "eps0": [I0.real, I0.imag],

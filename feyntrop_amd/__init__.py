@@ -97,6 +97,9 @@ def lib():
     L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_set_jtable_mode.argtypes = [vp, C.c_int]
+    L.ftr_set_specialized.argtypes = [vp, vp, C.c_int]
+    L.ftr_get_specialized.argtypes = [vp, C.POINTER(C.c_int)]
+    L.ftr_precompile_specialized.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_precompile_learned.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_learner_create.argtypes = [vp, vp, C.c_uint64, C.POINTER(vp)]
     L.ftr_learner_free.argtypes = [vp]
@@ -153,6 +156,15 @@ def precompile_learned(prob):
     inp = _Input(prob)
     err = C.create_string_buffer(8192)
     rc = lib().ftr_precompile_learned(C.byref(inp.s), err, 8192)
+    if rc:
+        raise FeyntropError(rc, err.value.decode(errors="replace"))
+
+
+def precompile_specialized(prob):
+    """Compile the kernel specialised to the problem's kinematic point."""
+    inp = _Input(prob)
+    err = C.create_string_buffer(8192)
+    rc = lib().ftr_precompile_specialized(C.byref(inp.s), err, 8192)
     if rc:
         raise FeyntropError(rc, err.value.decode(errors="replace"))
 
@@ -282,6 +294,17 @@ class Engine:
         rc = lib().ftr_set_variant(self.h, p.h, int(table_mode), int(block))
         if rc:
             raise self._err(rc)
+
+    def set_specialized(self, p, on=True):
+        """Specialise p's integration kernel to its kinematic point."""
+        rc = lib().ftr_set_specialized(self.h, p.h, 1 if on else 0)
+        if rc:
+            raise self._err(rc)
+
+    def specialized(self, p):
+        on = C.c_int()
+        lib().ftr_get_specialized(p.h, C.byref(on))
+        return bool(on.value)
 
     def set_jtable_mode(self, mode):
         """J-table construction: 0 auto, 1 host, 2 device."""

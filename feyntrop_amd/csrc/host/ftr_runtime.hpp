@@ -21,6 +21,7 @@ using ftrh::Error;
 
 struct Variant {
   int mode = 0, tmode = 0, block = 256, K = 1;
+  int cparams = 0; /* 1: the integration kernel is specialised to the kinematic point */
 };
 
 struct Module {
@@ -81,6 +82,8 @@ struct Geometry {
 /* kernel source of a problem's variant; `learned` adds the learned-proposal
  * kernels */
 std::string full_source(const ftrh::Prep& pr, const Variant& v, bool learned = false);
+/* the device parameters of a prepared problem */
+ftr::DevParams make_params(const ftrh::Prep& pr);
 /* code object for a source (hipRTC, cached on disk) */
 std::string code_object(const std::string& src, std::string* path_out);
 /* load (once per context) and resolve the kernels named `names` */

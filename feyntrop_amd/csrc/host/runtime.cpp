@@ -126,7 +126,33 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
     s << "};\n";
   };
   std::ostringstream s;
+  if (v.cparams) {
+    /* the kinematic point as compile-time constants (exact hex literals) */
+    const ftr::DevParams d = make_params(pr);
+    auto lit = [](double x) -> std::string {
+      if (std::isnan(x)) return "__builtin_nan(\"\")";
+      if (std::isinf(x)) return x > 0 ? "__builtin_inf()" : "-__builtin_inf()";
+      char b[64];
+      snprintf(b, sizeof b, "%a", x);
+      return b;
+    };
+    auto darr = [&](const char* name, const double* a, int n) {
+      s << "  static constexpr double " << name << "[" << n << "] = {";
+      for (int i = 0; i < n; i++) s << (i ? "," : "") << lit(a[i]);
+      s << "};\n";
+    };
+    s << "\nstruct FP {\n";
+    s << "  static constexpr double D = " << lit(d.D) << ", mhalfD = " << lit(d.mhalfD) << ", W = " << lit(d.W)
+      << ", mW = " << lit(d.mW) << ", Lf = " << lit(d.Lf) << ", IGtr = " << lit(d.IGtr) << ", def_ref = "
+      << lit(d.def_ref) << ", inv_dlw = " << lit(d.inv_dlw) << ", iD = " << lit(d.iD) << ", iW = " << lit(d.iW)
+      << ", use_iD = " << lit(d.use_iD) << ", use_iW = " << lit(d.use_iW) << ";\n";
+    darr("nu", d.nu, FTR_MAX_E);
+    darr("m2", d.m2, FTR_MAX_E);
+    darr("P", d.P, FTR_MAX_V * FTR_MAX_V);
+    s << "};\n";
+  }
   s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
+  s << (v.cparams ? "  using PRM = FP;\n" : "  using PRM = ftr::DevParams;\n");
   arr(s, "hi", g.hi);
   arr(s, "lo", g.lo);
   arr(s, "ord", ord);
@@ -238,9 +264,8 @@ std::string full_source(const ftrh::Prep& pr, const Variant& v, bool learned) {
   return src;
 }
 
-static void fill_params(ftr_problem* p) {
-  const ftrh::Prep& pr = p->prep;
-  ftr::DevParams& d = p->hprm;
+ftr::DevParams make_params(const ftrh::Prep& pr) {
+  ftr::DevParams d;
   memset(&d, 0, sizeof d);
   d.D = pr.D;
   d.mhalfD = -pr.D / 2.;
@@ -259,7 +284,10 @@ static void fill_params(ftr_problem* p) {
   }
   for (int i = 0; i < pr.g.V; i++)
     for (int j = 0; j < pr.g.V; j++) d.P[i * FTR_MAX_V + j] = pr.P[(size_t)i * pr.g.V + j];
+  return d;
 }
+
+static void fill_params(ftr_problem* p) { p->hprm = make_params(p->prep); }
 
 void prep_from_input(ftrh::Prep& pr, const ftr_input* in) {
   if (!in || in->E < 1) throw Error(FTR_ERR_INVALID, "input graph must have at least one edge");
@@ -426,6 +454,26 @@ int ftr_precompile(const ftr_input* in, char* err, int errlen) {
     /* both LDS limits a gfx950 part may report */
     for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
       const Variant v = choose_variant(pr, lim);
+      code_object(full_source(pr, v), nullptr);
+    }
+  } catch (std::exception& e) {
+    if (err && errlen > 0) {
+      strncpy(err, e.what(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fe ? fe->code : FTR_ERR_INVALID;
+  }
+  return FTR_OK;
+}
+
+int ftr_precompile_specialized(const ftr_input* in, char* err, int errlen) {
+  try {
+    ftrh::Prep pr;
+    prep_from_input(pr, in);
+    for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
+      Variant v = choose_variant(pr, lim);
+      v.cparams = 1;
       code_object(full_source(pr, v), nullptr);
     }
   } catch (std::exception& e) {
@@ -875,8 +923,9 @@ int ftr_set_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block) {
   if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
   try {
     HIPCHK(hipSetDevice(c->device));
-    for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock))
+    for (Variant v : candidate_variants(p->prep, c->prop.sharedMemPerBlock))
       if (v.tmode == table_mode && v.block == block) {
+        v.cparams = p->var.cparams;
         p->m = load_module(c, full_source(p->prep, v));
         p->var = v;
         p->lds = variant_lds(v, p->prep.g.E);
@@ -894,32 +943,56 @@ int ftr_set_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block) {
 int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block, double* best_ms) {
   if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
   const Variant keep = p->var;
-  double best = 1e300;
-  Variant bestv = keep;
-  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock)) {
-    if (ftr_set_variant(c, p, v.tmode, v.block) != FTR_OK) continue;
+  auto probe = [&]() {
     ftr_moments m;
     double t = 0;
     for (int rep = 0; rep < 2; rep++) {
-      if (ftr_integrate(c, p, n_probe, 12345, nullptr, &m) != FTR_OK) { t = 1e300; break; }
+      if (ftr_integrate(c, p, n_probe, 12345, nullptr, &m) != FTR_OK) return 1e300;
       t = m.kernel_ms;
     }
-    if (t < best) { best = t; bestv = v; }
+    return t;
+  };
+  double best = 1e300;
+  Variant bestv = keep;
+  /* (table placement, block) with the generic kernel ... */
+  if (ftr_set_specialized(c, p, 0) != FTR_OK) return FTR_ERR_DEVICE;
+  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock)) {
+    if (ftr_set_variant(c, p, v.tmode, v.block) != FTR_OK) continue;
+    const double t = probe();
+    if (t < best) {
+      best = t;
+      bestv = p->var;
+    }
   }
   if (best >= 1e300) {
     ftr_set_variant(c, p, keep.tmode, keep.block);
+    ftr_set_specialized(c, p, keep.cparams);
     return fail(c, FTR_ERR_DEVICE, "autotune: no variant ran");
   }
-  const int rc = ftr_set_variant(c, p, bestv.tmode, bestv.block);
+  int rc = ftr_set_variant(c, p, bestv.tmode, bestv.block);
   if (rc != FTR_OK) return rc;
   {
+    /* the placement and block are recorded per graph; specialisation to a
+     * kinematic point is not, since it costs a compile per point */
     mkdir(cache_dir().c_str(), 0755);
     std::ofstream f(tune_path(p->prep));
     if (f) f << bestv.tmode << " " << bestv.block << "\n";
   }
+  /* ... then the kernel specialised to this kinematic point */
+  if (ftr_set_specialized(c, p, 1) == FTR_OK) {
+    const double t = probe();
+    if (t < best) best = t;
+    else ftr_set_specialized(c, p, 0);
+  }
   if (table_mode) *table_mode = bestv.tmode;
   if (block) *block = bestv.block;
   if (best_ms) *best_ms = best;
+  return FTR_OK;
+}
+
+int ftr_get_specialized(const ftr_problem* p, int* on) {
+  if (!p || !on) return FTR_ERR_INVALID;
+  *on = p->var.cparams;
   return FTR_OK;
 }
 
@@ -931,6 +1004,23 @@ int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block) {
 }
 
 void ftr_free_string(char* s) { free(s); }
+
+int ftr_set_specialized(ftr_ctx* c, ftr_problem* p, int on) {
+  if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    Variant v = p->var;
+    v.cparams = on ? 1 : 0;
+    p->m = load_module(c, full_source(p->prep, v));
+    p->var = v;
+    if (p->lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return FTR_OK;
+}
 
 int ftr_set_jtable_mode(ftr_ctx* c, int mode) {
   if (!c || mode < 0 || mode > 2) return FTR_ERR_INVALID;

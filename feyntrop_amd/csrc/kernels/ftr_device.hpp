@@ -137,16 +137,20 @@ __host__ __device__ inline double dipow(double x, int n) {
   return n < 0 ? 1.0 / r : r;
 }
 /* x^(-D/2), x^(-W) with the integer fast path when it applies */
-__host__ __device__ inline double pow_mD(const DevParams& p, double x) {
+template <class PT>
+__host__ __device__ inline double pow_mD(const PT& p, double x) {
   return p.use_iD != 0.0 ? dipow(x, (int)p.iD) : pow(x, p.mhalfD);
 }
-__host__ __device__ inline double pow_mW(const DevParams& p, double x) {
+template <class PT>
+__host__ __device__ inline double pow_mW(const PT& p, double x) {
   return p.use_iW != 0.0 ? dipow(x, (int)p.iW) : pow(x, p.mW);
 }
-__host__ __device__ inline cplx cpow_mD(const DevParams& p, cplx z) {
+template <class PT>
+__host__ __device__ inline cplx cpow_mD(const PT& p, cplx z) {
   return p.use_iD != 0.0 ? cipow(z, (int)p.iD) : cpow_real(z, p.mhalfD);
 }
-__host__ __device__ inline cplx cpow_mW(const DevParams& p, cplx z) {
+template <class PT>
+__host__ __device__ inline cplx cpow_mW(const PT& p, cplx z) {
   return p.use_iW != 0.0 ? cipow(z, (int)p.iW) : cpow_real(z, p.mW);
 }
 
@@ -290,9 +294,9 @@ __host__ __device__ inline cplx ldlt_inplace(cplx (&M)[V * (V + 1) / 2], cplx (&
 /* tr(A^-1 P) over the trailing external block from the factors of A (packed
  * L below the diagonal, dinv); P entries at external vertices, the deleted
  * vertex vs excluded.  Only positions >= V - NEXT are touched. */
-template <class G, class T>
+template <class G, class T, class PT>
 __host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], const T (&dinv)[G::V],
-                                         const DevParams& p, int vs) {
+                                         const PT& p, int vs) {
   constexpr int V = G::V, N0 = G::V - G::NEXT;
   T tr = T{};
 #pragma unroll
@@ -325,13 +329,17 @@ __host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], co
 /* log of the projective rescale factor (feynman_integral.hpp:201-207):
  * scale = (psi_tr^(-D/2) (psi_tr/xi_tr)^W)^(1/(D L/2 + W)), from
  * log psi_tr and log G (xi_tr = G psi_tr). */
-__host__ __device__ inline double log_scale(const DevParams& p, double lpsi, double lG) {
+template <class PT>
+__host__ __device__ inline double log_scale(const PT& p, double lpsi, double lG) {
   const double t = p.mhalfD * lpsi - (p.W != 0.0 ? p.W * lG : 0.0);
   return p.inv_dlw * t;
 }
 
-template <class G, int K, int MODE>
-__host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E], double lscale,
+/* PT: the problem's parameters, ftr::DevParams read from memory or a type
+ * with the same members as compile-time constants (kernels specialised to
+ * one kinematic point). */
+template <class G, int K, int MODE, class PT = DevParams>
+__host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double lscale,
                                           cplx (&out)[K]) {
   constexpr int E = G::E, V = G::V, NT = V * (V + 1) / 2;
   /* vertex weights and v* (first minimum) -- feynman_integral.hpp:166-194 */
@@ -392,7 +400,7 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
       out[0] = cmk(R, 0.0);
       return true;
     }
-    const double tr = trace_inv_P<G, double>(M, dinv, p, vs);
+    const double tr = trace_inv_P<G, double, PT>(M, dinv, p, vs);
     double dot = 0.0;
 #pragma unroll
     for (int e = 0; e < E; e++) dot += p.m2[e] * X[e];
@@ -546,7 +554,7 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
         for (int j = 0; j <= i; j++)
           if (G::ord[i] == vs || G::ord[j] == vs) CM[FTR_TRI(i, j)] = cmk((i == j) ? 1.0 : 0.0, 0.0);
       cdet = ldlt_inplace<V>(CM, cdinv);
-      ctr = trace_inv_P<G, cplx>(CM, cdinv, p, vs);
+      ctr = trace_inv_P<G, cplx, PT>(CM, cdinv, p, vs);
     }
     cplx cxp = cmk(1.0, 0.0), cdot = cmk(0.0, 0.0);
 #pragma unroll
@@ -579,6 +587,19 @@ __host__ __device__ inline bool integrand(const DevParams& p, double (&X)[G::E],
   }
 }
 #undef FTR_SYM
+
+/* The integration kernel's parameters: read from memory (DevParams), or,
+ * for kernels specialised to one kinematic point, a type whose members are
+ * compile-time constants (no storage). */
+template <class PT>
+struct ParamsFor {
+  PT c;
+  __device__ const PT& get(const Launch&) const { return c; }
+};
+template <>
+struct ParamsFor<DevParams> {
+  __device__ const DevParams& get(const Launch& a) const { return *a.prm; }
+};
 
 /* -------------------------------------------------------- table views -- */
 /* TMODE 0: whole table in LDS; 1: w = J/r in LDS, the rest from global
@@ -660,7 +681,9 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   extern __shared__ __align__(16) char lds[];
   constexpr int E = G::E;
   constexpr int NQ = 2 + 4 * K;
-  const DevParams& p = *a.prm;
+  using PT = typename G::PRM;
+  const ParamsFor<PT> ps{};
+  const PT& p = ps.get(a);
   TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
   double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
 
@@ -676,7 +699,7 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
     double X[E], lpsi, lG;
     sample_point<G>(rng, tv, X, lpsi, lG);
     cplx out[K];
-    if (!integrand<G, K, MODE>(p, X, log_scale(p, lpsi, lG), out)) {
+    if (!integrand<G, K, MODE, PT>(p, X, log_scale(p, lpsi, lG), out)) {
       acc[1] += 1.0;
       continue;
     }

@@ -9,7 +9,7 @@ import concurrent.futures as cf
 import os
 import sys
 
-from feyntrop_amd import host_path, precompile, precompile_learned, workloads
+from feyntrop_amd import host_path, precompile, precompile_learned, precompile_specialized, workloads
 
 
 def problems():
@@ -27,6 +27,8 @@ def _one(item):
         precompile(prob)
         if learned and len(prob["graph"]) <= 16:
             precompile_learned(prob)
+        if name in workloads.CONFIGS:   # bench workloads: the kinematics-specialised kernel too
+            precompile_specialized(prob)
         return name, None
     except Exception as e:  # report, keep going
         return name, str(e)

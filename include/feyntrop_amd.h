@@ -135,12 +135,23 @@ int ftr_integrate_batch(ftr_ctx* ctx, int n, ftr_problem* const* probs, const ui
 /* Autotuner (no reference counterpart; replaces the reference's fixed OpenMP
  * chunking): times every feasible (J-table placement, block size) kernel
  * variant on n_probe samples, keeps the fastest for the problem and records
- * it in the cache directory, where later ftr_prepare calls pick it up.
+ * it in the cache directory, where later ftr_prepare calls pick it up; then
+ * times the kernel specialised to the problem's kinematic point
+ * (ftr_set_specialized) and keeps it for this problem if it is faster.
  * table_mode: 0 whole J-table in LDS, 1 J/r in LDS + rest from L2/HBM,
  * 2 all from L2/HBM. */
 int ftr_autotune(ftr_ctx* ctx, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block,
                  double* best_kernel_ms);
 int ftr_set_variant(ftr_ctx* ctx, ftr_problem* p, int table_mode, int block);
+/* Specialise the problem's integration kernel to its kinematic point (no
+ * reference counterpart): D, P, m^2, nu, lambda and the exponents become
+ * compile-time constants of the generated kernel (one hipRTC compile per
+ * kinematic point, cached), on = 0 returns to the generic kernel.  Same
+ * samples, same estimator; for long runs of one point. */
+int ftr_set_specialized(ftr_ctx* ctx, ftr_problem* p, int on);
+int ftr_get_specialized(const ftr_problem* p, int* on);
+/* Compile the specialised kernel of a problem into the cache without a GPU. */
+int ftr_precompile_specialized(const ftr_input* in, char* err, int errlen);
 int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block);
 
 /* Where ftr_prepare builds the 2^E-entry J-table (tropical_sampling.hpp:

@@ -162,6 +162,30 @@ def test_estimate_equals_reference_one_thread(engine, golden_problems, name):
             assert abs(s2[k, q] - rs2) <= 1e-9 * abs(rs2) + 1e-300, (k, q)
 
 
+@pytest.mark.parametrize("name", ["wheel_k4", "double_box", "triple_ladder", "2L_3pt_tutorial"])
+def test_specialized_kernel_equals_reference_one_thread(engine, golden_problems, name):
+    """The kernel specialised to the kinematic point (constants folded at
+    compile time) draws the same samples: the reference's one-thread moments
+    (same bars as the generic kernel)."""
+    print(f"[specialized] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    engine.set_specialized(p, True)
+    assert engine.specialized(p)
+    g = golden_problems[name]["estimate"]
+    N = g["N"]
+    ref = _hex(g["moments"]).reshape(p.K, 2, 3)
+    m = engine.integrate(p, N, 0)
+    n, s1, s2 = m.arrays()
+    assert m.dropped == 0
+    for k in range(p.K):
+        assert n[k, 0] == ref[k, 0, 0]
+        for q in ((0, 1) if p.info.path == 2 else (0,)):
+            rn, rs1, rs2 = ref[k, q]
+            sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+            assert abs(s1[k, q] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn), (k, q)
+            assert abs(s2[k, q] - rs2) <= 1e-9 * abs(rs2) + 1e-300, (k, q)
+
+
 @pytest.mark.parametrize("name", ["wheel_k4", "double_box", "2L_3pt_tutorial"])
 def test_estimate_independent_of_launch_geometry(engine, name):
     """The multiset of samples depends only on (N, seed): two lane counts
