@@ -30,12 +30,15 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
 # HBM traffic of one ftrk_int launch (N = 1e8) from two rocprofv3 --pmc runs of
-# this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean of the
-# timed launches), each scaled by what the same counter read for a 256 MiB
-# device-to-device copy in the same run (FETCH 131086.8 KiB, WRITE 262144 KiB
-# for 262144 KiB moved).  Summary and command under profiles/.
+# this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, timed launches),
+# each scaled by what the same counter read for a 256 MiB device-to-device
+# copy in the same run (FETCH 131086.9 KiB, WRITE 262144 KiB for 262144 KiB
+# moved).  Per kernel kind: generic (parameters read from memory) and
+# specialised to the kinematic point.  Summary and commands under profiles/.
+_FETCH_SCALE = 262144 / 131086.9375
 PMC_TRAFFIC_BYTES = {
-    "double_box": (457.458 * (262144 / 131086.8125) + 24.0 * (262144 / 262144)) * 1024,
+    ("double_box", "generic"): (457.938 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "specialised"): (412.0 * _FETCH_SCALE + 24.0) * 1024,
 }
 
 
@@ -310,7 +313,8 @@ def main():
         "eps0": [I0.real, I0.imag],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": PMC_TRAFFIC_BYTES.get(args.workload) if N == 10**8 else None,
+                     "traffic": PMC_TRAFFIC_BYTES.get((args.workload, "specialised" if eng.specialized(p) else
+                                                       "generic")) if N == 10**8 else None,
                      "kernel": "ftrk_int", "flops_per_sample": fps, "kernel_ms_mean": mean_kms},
         "learned": None,
     }
