@@ -94,6 +94,11 @@ __host__ __device__ inline cplx operator*(cplx a, cplx b) {
   return cplx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 __host__ __device__ inline cplx operator*(double s, cplx a) { return cplx{s * a.re, s * a.im}; }
+/* y - a b: four FMAs for complex operands, one for real ones */
+__host__ __device__ inline cplx fms(cplx y, cplx a, cplx b) {
+  return cplx{fma(-a.re, b.re, fma(a.im, b.im, y.re)), fma(-a.re, b.im, fma(-a.im, b.re, y.im))};
+}
+__host__ __device__ inline double fms(double y, double a, double b) { return fma(-a, b, y); }
 __host__ __device__ inline cplx crecip(cplx a) {
   const double inv = 1.0 / (a.re * a.re + a.im * a.im);
   return cplx{a.re * inv, -a.im * inv};
@@ -283,7 +288,7 @@ __host__ __device__ inline cplx ldlt_inplace(cplx (&M)[V * (V + 1) / 2], cplx (&
     for (int i = k + 1; i < V; i++) {
       const cplx lik = M[FTR_TRI(i, k)] * inv;
 #pragma unroll
-      for (int j = k + 1; j <= i; j++) M[FTR_TRI(i, j)] = M[FTR_TRI(i, j)] - lik * M[FTR_TRI(j, k)];
+      for (int j = k + 1; j <= i; j++) M[FTR_TRI(i, j)] = fms(M[FTR_TRI(i, j)], lik, M[FTR_TRI(j, k)]);
     }
 #pragma unroll
     for (int i = k + 1; i < V; i++) M[FTR_TRI(i, k)] = M[FTR_TRI(i, k)] * inv;
@@ -309,13 +314,13 @@ __host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], co
 #pragma unroll
     for (int i = N0; i < V; i++)
 #pragma unroll
-      for (int q = pj; q < i; q++) y[i] = y[i] - M[FTR_TRI(i, q)] * y[q];
+      for (int q = pj; q < i; q++) y[i] = fms(y[i], M[FTR_TRI(i, q)], y[q]);
 #pragma unroll
     for (int i = N0; i < V; i++) y[i] = y[i] * dinv[i];
 #pragma unroll
     for (int i = V - 1; i >= N0; i--)
 #pragma unroll
-      for (int q = i + 1; q < V; q++) y[i] = y[i] - M[FTR_TRI(q, i)] * y[q];
+      for (int q = i + 1; q < V; q++) y[i] = fms(y[i], M[FTR_TRI(q, i)], y[q]);
     T colsum = T{};
 #pragma unroll
     for (int b = 0; b < G::NEXT; b++) colsum = colsum + p.P[G::ord[N0 + b] * FTR_MAX_V + vj] * y[N0 + b];
@@ -507,7 +512,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
         for (int k = 0; k < j; k++) w[k] = d[k] * L[FTR_TRI(j, k)];
         cplx dj = cmk(Xi[j], p.def_ref * dd[FTR_TRI(j, j)]);
 #pragma unroll
-        for (int k = 0; k < j; k++) dj = dj - L[FTR_TRI(j, k)] * w[k];
+        for (int k = 0; k < j; k++) dj = fms(dj, L[FTR_TRI(j, k)], w[k]);
         d[j] = dj;
         jdet = jdet * dj;
         const cplx inv = crecip(dj);
@@ -515,7 +520,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
         for (int i = j + 1; i < E; i++) {
           cplx v = cmk(0.0, p.def_ref * dd[FTR_TRI(i, j)]);
 #pragma unroll
-          for (int k = 0; k < j; k++) v = v - L[FTR_TRI(i, k)] * w[k];
+          for (int k = 0; k < j; k++) v = fms(v, L[FTR_TRI(i, k)], w[k]);
           L[FTR_TRI(i, j)] = v * inv;
         }
       }
