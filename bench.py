@@ -87,20 +87,25 @@ def cpu_baseline(prob, n_samples, threads):
 
 def run_scan(eng, n_points, seed=0):
     """Batched scan: n_points kinematic points (1- to 3-loop topologies,
-    Euclidean and Minkowski, N log-uniform in [1e5, 1e7]) in one hipGraph of
-    multi-problem kernels.  Returns throughput over the whole batch."""
+    Euclidean and Minkowski, N log-uniform in [1e5, 1e7]) prepared together
+    (ftr_prepare_batch: host phases in parallel) and integrated in one
+    hipGraph of multi-problem kernels.  value = samples / integration wall
+    time; end_to_end_value also counts the preparation."""
     import feyntrop_amd
     from feyntrop_amd import workloads as W
-    probs = []
-
-    def accept(prob):
-        try:
-            probs.append(eng.prepare(prob))
-            return True
-        except feyntrop_amd.FeyntropError:
-            return False
+    cand = W.scan_points(n_points * 2, seed)          # candidates; the rejected ones are skipped
+    pts, probs = [], []
     t0 = time.perf_counter()
-    pts = W.scan_points(n_points, seed, accept=accept)
+    n_cand = 0
+    for c0 in range(0, len(cand), 1024):               # prepare in chunks until n_points are accepted
+        chunk = cand[c0:c0 + 1024]
+        n_cand += len(chunk)
+        for pt, q in zip(chunk, eng.prepare_batch(chunk)):
+            if not isinstance(q, Exception) and len(probs) < n_points:
+                pts.append(pt)
+                probs.append(q)
+        if len(probs) >= n_points:
+            break
     t_prep = time.perf_counter() - t0
     Ns = [p["N"] for p in pts]
     seeds = [p["seed"] for p in pts]
@@ -114,52 +119,9 @@ def run_scan(eng, n_points, seed=0):
         paths[q.info.path] = paths.get(q.info.path, 0) + 1
     return {"workload": f"batched scan: {len(pts)} points, {len(set(p['_topology'] for p in pts))} topologies",
             "samples": tot, "value": tot / wall, "unit": "samples/s", "graph_ms": outs[0].kernel_ms,
-            "wall_s": wall, "prep_s": t_prep, "paths": paths,
+            "wall_s": wall, "prep_s": t_prep, "prepared_candidates": n_cand,
+            "end_to_end_value": tot / (wall + t_prep), "paths": paths,
             "dropped": int(sum(m.dropped for m in outs))}
-
-
-def run_large(eng, name="ladder7", steps=2):
-    """A 22-edge planar ladder: the 2^22-entry J-table built on the device
-    (timed against the host pass), then plain sampling with the tables read
-    from L2/HBM."""
-    from feyntrop_amd import workloads as W
-    prob = W.LARGE[name]()
-    out = {}
-    for mode, tag in ((2, "device"), (1, "host")):
-        eng.set_jtable_mode(mode)
-        t0 = time.perf_counter()
-        p = eng.prepare(prob)
-        out[f"prepare_s_{tag}_jtable"] = time.perf_counter() - t0
-    eng.set_jtable_mode(0)
-    N = int(prob["N"])
-    eng.integrate(p, 1_000_000, 1)
-    t0 = time.perf_counter()
-    ms = [eng.integrate(p, N, 100 + i).kernel_ms for i in range(steps)]
-    wall = time.perf_counter() - t0
-    fps = flops_per_sample(p.info)
-    out.update({"workload": f"{name}: E={p.E} V={p.V} K={p.K}, 2^{p.E} J-table entries",
-                "value": N * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
-                "roofline_frac": fps * N / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS})
-    return out
-
-
-def run_tuned(eng, name, steps=2):
-    """A named workload with the autotuner's kernel variant."""
-    from feyntrop_amd import workloads as W
-    prob = W.all_problems()[name]()
-    p = eng.prepare(prob)
-    tm, blk, _ = eng.autotune(p, n_probe=min(int(prob["N"]), 20_000_000))
-    eng.integrate(p, prob["N"], 1)
-    t0 = time.perf_counter()
-    ms = []
-    for i in range(steps):
-        ms.append(eng.integrate(p, prob["N"], 100 + i).kernel_ms)
-    wall = time.perf_counter() - t0
-    fps = flops_per_sample(p.info)
-    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}", "N_per_step": prob["N"],
-            "value": prob["N"] * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
-            "variant": {"table_mode": tm, "block": blk},
-            "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
 
 
 def _rel_err(p, m):

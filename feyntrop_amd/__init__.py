@@ -97,6 +97,9 @@ def lib():
     L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_set_jtable_mode.argtypes = [vp, C.c_int]
+    L.ftr_prepare_batch.argtypes = [vp, C.c_int, C.POINTER(ftr_input), C.POINTER(vp), C.POINTER(C.c_int)]
+    L.ftr_batch_error.restype = C.c_char_p
+    L.ftr_batch_error.argtypes = [vp, C.c_int]
     L.ftr_set_specialized.argtypes = [vp, vp, C.c_int]
     L.ftr_get_specialized.argtypes = [vp, C.POINTER(C.c_int)]
     L.ftr_precompile_specialized.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
@@ -251,6 +254,25 @@ class Engine:
         if rc:
             raise self._err(rc)
         return Problem(self, out, prob)
+
+    def prepare_batch(self, probs):
+        """Prepare many problems (host phases in parallel).  Returns a list with
+        a Problem, or a FeyntropError for a rejected problem, per input."""
+        n = len(probs)
+        inps = [_Input(pr) for pr in probs]
+        arr = (ftr_input * n)(*[x.s for x in inps])
+        outs = (C.c_void_p * n)()
+        st = (C.c_int * n)()
+        rc = lib().ftr_prepare_batch(self.h, n, arr, outs, st)
+        if rc:
+            raise self._err(rc)
+        res = []
+        for i in range(n):
+            if st[i]:
+                res.append(FeyntropError(st[i], lib().ftr_batch_error(self.h, i).decode(errors="replace")))
+            else:
+                res.append(Problem(self, C.c_void_p(outs[i]), probs[i]))
+        return res
 
     def integrate(self, p, N, seed=0, lanes=0):
         m = ftr_moments()

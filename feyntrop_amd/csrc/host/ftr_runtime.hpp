@@ -47,6 +47,7 @@ struct ftr_ctx {
   double* h_pinned = nullptr; /* pinned copy-back buffer of the batch graph */
   size_t pinned_bytes = 0;
   int jtable_mode = 0;        /* 0 auto, 1 host, 2 device (ftr_set_jtable_mode) */
+  std::vector<std::string> batch_errors; /* messages of ftr_prepare_batch */
 };
 
 struct ftr_problem {
@@ -57,6 +58,7 @@ struct ftr_problem {
   size_t lds = 0;
   Module m;
   ftr::DevParams hprm;
+  void* d_slab = nullptr; /* one allocation: parameters and sampler tables */
   ftr::DevParams* d_prm = nullptr;
   double *d_w = nullptr, *d_jr = nullptr, *d_ir = nullptr;
   uint8_t* d_mml = nullptr;

@@ -487,6 +487,57 @@ int ftr_precompile_specialized(const ftr_input* in, char* err, int errlen) {
   return FTR_OK;
 }
 
+/* Host phase of preparation: validation, regime, J-table (host or device). */
+static void prepare_host(ftr_ctx* c, ftr_problem* p, const ftr_input* in) {
+  if (c && c->jtable_mode != 1) {
+    /* the J-table on the device: always (mode 2) or from E = 13 on (auto) */
+    const int mode = c->jtable_mode;
+    p->prep.jtable_fn = [c, mode](ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& r) {
+      if (mode == 2 || pr.g.E >= FTR_DEVICE_JTABLE_MIN_E) device_jtable(c, pr, js, r);
+      else ftrh::host_jtable(pr, js, r);
+    };
+  }
+  prep_from_input(p->prep, in);
+}
+
+/* Device phase: kernel module, and the parameters and sampler tables in one
+ * allocation (DevParams | w | J | 1/r | mm|L<<1), uploaded with one copy
+ * from a host staging buffer. */
+static void prepare_device(ftr_ctx* c, ftr_problem* p) {
+  HIPCHK(hipSetDevice(c->device));
+  if (!load_tuned(p->prep, c->prop.sharedMemPerBlock, &p->var))
+    p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock);
+  p->lds = variant_lds(p->var, p->prep.g.E);
+  p->m = load_module(c, full_source(p->prep, p->var));
+  if (p->lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
+  fill_params(p);
+  const size_t NT = (size_t)1 << p->prep.g.E;
+  const size_t o_w = (sizeof(ftr::DevParams) + 255) & ~(size_t)255;
+  const size_t o_jr = o_w + NT * 8, o_ir = o_jr + NT * 8, o_mml = o_ir + NT * 8, total = o_mml + NT;
+  std::vector<char> h(total);
+  memcpy(h.data(), &p->hprm, sizeof(ftr::DevParams));
+  double* w = (double*)(h.data() + o_w);
+  double* ir = (double*)(h.data() + o_ir);
+  uint8_t* mml = (uint8_t*)(h.data() + o_mml);
+  memcpy(h.data() + o_jr, p->prep.Jr.data(), NT * 8);
+  for (size_t i = 0; i < NT; i++) {
+    w[i] = p->prep.Jr[i] / p->prep.r[i];
+    ir[i] = 1. / p->prep.r[i];
+    mml[i] = (uint8_t)((p->prep.mm[i] ? 1 : 0) | (p->prep.Lt[i] << 1));
+  }
+  char* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, total));
+  p->d_slab = d;
+  p->d_prm = (ftr::DevParams*)d;
+  p->d_w = (double*)(d + o_w);
+  p->d_jr = (double*)(d + o_jr);
+  p->d_ir = (double*)(d + o_ir);
+  p->d_mml = (uint8_t*)(d + o_mml);
+  HIPCHK(hipMemcpyHtoDAsync(d, h.data(), total, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
   if (!out) return FTR_ERR_INVALID;
   *out = nullptr;
@@ -495,48 +546,8 @@ int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
   p->device = c ? c->device : 0;
   try {
     const auto t0 = std::chrono::steady_clock::now();
-    if (c && c->jtable_mode != 1) {
-      /* the J-table on the device: always (mode 2) or from E = 13 on (auto) */
-      const int mode = c->jtable_mode;
-      p->prep.jtable_fn = [c, mode](ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& r) {
-        if (mode == 2 || pr.g.E >= FTR_DEVICE_JTABLE_MIN_E) device_jtable(c, pr, js, r);
-        else ftrh::host_jtable(pr, js, r);
-      };
-    }
-    prep_from_input(p->prep, in);
-    if (!c) {
-      /* host-only preparation: validation, regime, J-table */
-      p->prep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      *out = p;
-      return FTR_OK;
-    }
-    HIPCHK(hipSetDevice(c->device));
-    if (!load_tuned(p->prep, c->prop.sharedMemPerBlock, &p->var))
-      p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock);
-    p->lds = variant_lds(p->var, p->prep.g.E);
-    p->m = load_module(c, full_source(p->prep, p->var));
-    if (p->lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
-    fill_params(p);
-    const size_t NT = (size_t)1 << p->prep.g.E;
-    std::vector<double> w(NT), ir(NT);
-    std::vector<uint8_t> mml(NT);
-    for (size_t i = 0; i < NT; i++) {
-      w[i] = p->prep.Jr[i] / p->prep.r[i];
-      ir[i] = 1. / p->prep.r[i];
-      mml[i] = (uint8_t)((p->prep.mm[i] ? 1 : 0) | (p->prep.Lt[i] << 1));
-    }
-    HIPCHK(hipMalloc((void**)&p->d_prm, sizeof(ftr::DevParams)));
-    HIPCHK(hipMalloc((void**)&p->d_w, NT * 8));
-    HIPCHK(hipMalloc((void**)&p->d_jr, NT * 8));
-    HIPCHK(hipMalloc((void**)&p->d_ir, NT * 8));
-    HIPCHK(hipMalloc((void**)&p->d_mml, NT));
-    HIPCHK(hipMemcpyHtoDAsync(p->d_prm, &p->hprm, sizeof(ftr::DevParams), c->stream));
-    HIPCHK(hipMemcpyHtoDAsync(p->d_w, w.data(), NT * 8, c->stream));
-    HIPCHK(hipMemcpyHtoDAsync(p->d_jr, p->prep.Jr.data(), NT * 8, c->stream));
-    HIPCHK(hipMemcpyHtoDAsync(p->d_ir, ir.data(), NT * 8, c->stream));
-    HIPCHK(hipMemcpyHtoDAsync(p->d_mml, mml.data(), NT, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    prepare_host(c, p, in);
+    if (c) prepare_device(c, p);
     p->prep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   } catch (std::exception& e) {
     const Error* fe = dynamic_cast<const Error*>(&e);
@@ -549,15 +560,76 @@ int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
   return FTR_OK;
 }
 
+int ftr_prepare_batch(ftr_ctx* c, int n, const ftr_input* in, ftr_problem** out, int* status) {
+  if (!c || n < 0 || (n > 0 && (!in || !out || !status))) return FTR_ERR_INVALID;
+  c->batch_errors.assign(n, std::string());
+  std::vector<ftr_problem*> ps(n, nullptr);
+  for (int i = 0; i < n; i++) {
+    out[i] = nullptr;
+    status[i] = FTR_OK;
+    ps[i] = new ftr_problem();
+    ps[i]->ctx = c;
+    ps[i]->device = c->device;
+  }
+  /* host phases in parallel over problems (each problem's own J-table pass
+   * then runs on one thread); the device J-table, when chosen, is issued
+   * from one thread at a time */
+  std::mutex dev_mu;
+  const int mode = c->jtable_mode;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n; i++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+      ftr_problem* p = ps[i];
+      if (mode != 1)
+        p->prep.jtable_fn = [c, mode, &dev_mu](ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& r) {
+          if (mode == 2 || pr.g.E >= FTR_DEVICE_JTABLE_MIN_E) {
+            std::lock_guard<std::mutex> lk(dev_mu);
+            device_jtable(c, pr, js, r);
+          } else {
+            ftrh::host_jtable(pr, js, r);
+          }
+        };
+      prep_from_input(p->prep, &in[i]);
+      p->prep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } catch (std::exception& e) {
+      const Error* fe = dynamic_cast<const Error*>(&e);
+      status[i] = fe ? fe->code : FTR_ERR_INVALID;
+      c->batch_errors[i] = e.what();
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    if (status[i] != FTR_OK) {
+      ftr_problem_free(ps[i]);
+      continue;
+    }
+    try {
+      const auto t0 = std::chrono::steady_clock::now();
+      prepare_device(c, ps[i]);
+      ps[i]->prep.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      out[i] = ps[i];
+    } catch (std::exception& e) {
+      const Error* fe = dynamic_cast<const Error*>(&e);
+      status[i] = fe ? fe->code : FTR_ERR_DEVICE;
+      c->batch_errors[i] = e.what();
+      ftr_problem_free(ps[i]);
+    }
+  }
+  return FTR_OK;
+}
+
+const char* ftr_batch_error(const ftr_ctx* c, int i) {
+  if (!c || i < 0 || i >= (int)c->batch_errors.size()) return "";
+  return c->batch_errors[i].c_str();
+}
+
 void ftr_problem_free(ftr_problem* p) {
   if (!p) return;
   /* the context may already be gone: only the device id is used */
-  if (p->d_prm) (void)hipSetDevice(p->device);
-  if (p->d_prm) (void)hipFree(p->d_prm);
-  if (p->d_w) (void)hipFree(p->d_w);
-  if (p->d_jr) (void)hipFree(p->d_jr);
-  if (p->d_ir) (void)hipFree(p->d_ir);
-  if (p->d_mml) (void)hipFree(p->d_mml);
+  if (p->d_slab) {
+    (void)hipSetDevice(p->device);
+    (void)hipFree(p->d_slab);
+  }
   delete p;
 }
 

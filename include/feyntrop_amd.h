@@ -100,6 +100,13 @@ const char* ftr_last_error(const ftr_ctx* ctx);
  * must be freed before its context is destroyed. */
 int ftr_prepare(ftr_ctx* ctx, const ftr_input* in, ftr_problem** out);
 void ftr_problem_free(ftr_problem* p);
+/* Prepare n problems at once (no reference counterpart; for scans): the host
+ * phases (validation, regime, J-table) run in parallel over the problems,
+ * then each problem gets its kernel and one device allocation.  out[i] is
+ * NULL and status[i] nonzero for a rejected problem, whose message
+ * ftr_batch_error(ctx, i) returns (valid until the next batch call). */
+int ftr_prepare_batch(ftr_ctx* ctx, int n, const ftr_input* in, ftr_problem** out, int* status);
+const char* ftr_batch_error(const ftr_ctx* ctx, int i);
 int ftr_problem_info(const ftr_problem* p, ftr_info* info);
 /* J-table: arrays of 2^E entries (Jr, r, mm, L). */
 int ftr_jtable(const ftr_problem* p, double* Jr, double* r, uint8_t* mm, uint8_t* L);

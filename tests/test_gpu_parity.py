@@ -276,6 +276,30 @@ def test_cli_batch_mode_equals_single_runs():
         np.testing.assert_allclose(a[..., 1], b[..., 1], rtol=1e-8, atol=1e-14)
 
 
+def test_prepare_batch_equals_individual(engine):
+    """ftr_prepare_batch (host phases in parallel): every accepted problem has
+    the J-table, flags and estimate of its individual preparation; every
+    rejected one the same message."""
+    import feyntrop_amd
+    cand = W.scan_points(96, seed=5)
+    batch = engine.prepare_batch(cand)
+    n_ok = 0
+    for prob, qb in zip(cand, batch):
+        try:
+            qi = engine.prepare(prob)
+        except feyntrop_amd.FeyntropError as e:
+            assert isinstance(qb, feyntrop_amd.FeyntropError) and str(qb) == str(e)
+            continue
+        assert not isinstance(qb, Exception), qb
+        n_ok += 1
+        for a, b in zip(qi.jtable(), qb.jtable()):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+        assert qi.info.IGtr == qb.info.IGtr and qi.info.path == qb.info.path
+        mi, mb = engine.integrate(qi, 20000, 3), engine.integrate(qb, 20000, 3)
+        assert mi.arrays()[1].tolist() == mb.arrays()[1].tolist()
+    assert n_ok > 40
+
+
 def test_batch_equals_individual(engine):
     """The batched scan draws the same samples as one call per problem: the
     same counts, sums equal up to summation order."""
