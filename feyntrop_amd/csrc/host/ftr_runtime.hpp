@@ -59,6 +59,9 @@ struct ftr_problem {
   Module m;
   ftr::DevParams hprm;
   void* d_slab = nullptr; /* one allocation: parameters and sampler tables */
+  void* d_rows_slab = nullptr; /* table mode 3: rows and state records */
+  double* d_rows = nullptr;
+  ftr::StateRec* d_rec = nullptr;
   ftr::DevParams* d_prm = nullptr;
   double *d_w = nullptr, *d_jr = nullptr, *d_ir = nullptr;
   uint8_t* d_mml = nullptr;
@@ -94,11 +97,17 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
                       int nq, hipFunction_t occ_fn);
 void ensure_scratch(ftr_ctx* c, size_t bytes);
 void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
-Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit);
+Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit = (size_t)128 << 30);
 size_t variant_lds(const Variant& v, int E);
 int fail(ftr_ctx* c, int code, const std::string& msg);
 /* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */
 void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& res);
+/* build the row tables of table mode 3 on the device (jtable_gpu.cpp) */
+void device_rows(ftr_ctx* c, ftr_problem* p);
+/* bytes of the table-mode-3 tables of a graph with E edges */
+inline size_t rows_bytes(int E) { return ((size_t)1 << E) * ((size_t)8 * E + sizeof(ftr::StateRec)); }
+/* the table pointers of a problem into a launch */
+void fill_tables(ftr::Launch& a, const ftr_problem* p);
 /* smallest E for which ftr_prepare builds the J-table on the device in auto mode */
 constexpr int FTR_DEVICE_JTABLE_MIN_E = 13;
 extern thread_local std::string g_host_err;

@@ -214,11 +214,7 @@ ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
   ftr_problem* p = l->p;
   ftr::LearnLaunch L;
   memset(&L, 0, sizeof L);
-  L.a.prm = p->d_prm;
-  L.a.t_w = p->d_w;
-  L.a.t_jr = p->d_jr;
-  L.a.t_ir = p->d_ir;
-  L.a.t_mmL = p->d_mml;
+  fill_tables(L.a, p);
   L.a.block_state = G.d_states;
   L.a.lane_poly = G.d_polys;
   L.a.N = N;

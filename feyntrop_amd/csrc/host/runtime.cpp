@@ -62,7 +62,7 @@ static uint64_t fnv1a(const std::string& s) {
 }
 
 /* Kernel variant for a problem: path -> MODE, table size -> placement. */
-Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
+Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit) {
   Variant v;
   v.K = pr.K;
   v.mode = pr.path;
@@ -77,6 +77,8 @@ Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
      * big one on the real path (the deformed path needs > 128 VGPRs) */
     v.tmode = 1;
     if (pr.path != 2 && wonly + 16 * (2 + 4 * pr.K) * 8 <= lds_limit) v.block = 1024;
+  } else if (rows_bytes(E) <= mem_limit) {
+    v.tmode = 3; /* rows in HBM: one contiguous read per cut step */
   } else {
     v.tmode = 2;
   }
@@ -84,15 +86,17 @@ Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit) {
 }
 
 /* Feasible (table placement, block size) pairs for the autotuner. */
-static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_limit) {
+static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_limit,
+                                               size_t mem_limit = (size_t)128 << 30) {
   std::vector<Variant> out;
   const int E = pr.g.E;
   const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
-  for (int tm = 0; tm < 3; tm++)
+  for (int tm = 0; tm < 4; tm++)
     for (int blk : {256, 512, 1024}) {
       const size_t t = tm == 0 ? full : (tm == 1 ? wonly : 0);
       if (t + (size_t)(blk / 64) * (2 + 4 * pr.K) * 8 + 16 > lds_limit) continue;
       if (tm == 0 && full > 64 * 1024 && blk == 256) continue;
+      if (tm == 3 && (E < 10 || rows_bytes(E) > mem_limit)) continue;
       Variant v;
       v.K = pr.K;
       v.mode = pr.path;
@@ -395,6 +399,19 @@ int fail(ftr_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+void fill_tables(ftr::Launch& a, const ftr_problem* p) {
+  a.prm = p->d_prm;
+  a.t_w = p->d_w;
+  a.t_jr = p->d_jr;
+  a.t_ir = p->d_ir;
+  a.t_mmL = p->d_mml;
+  a.t_rows = p->d_rows;
+  a.t_rec = p->d_rec;
+}
+
+/* device memory available for table mode 3 (half of the device) */
+static size_t rows_mem_limit(const ftr_ctx* c) { return c->prop.totalGlobalMem / 2; }
+
 /* --------------------------------------------------------------- C ABI -- */
 extern "C" {
 
@@ -506,7 +523,7 @@ static void prepare_host(ftr_ctx* c, ftr_problem* p, const ftr_input* in) {
 static void prepare_device(ftr_ctx* c, ftr_problem* p) {
   HIPCHK(hipSetDevice(c->device));
   if (!load_tuned(p->prep, c->prop.sharedMemPerBlock, &p->var))
-    p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock);
+    p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c));
   p->lds = variant_lds(p->var, p->prep.g.E);
   p->m = load_module(c, full_source(p->prep, p->var));
   if (p->lds > 64 * 1024)
@@ -536,6 +553,7 @@ static void prepare_device(ftr_ctx* c, ftr_problem* p) {
   p->d_mml = (uint8_t*)(d + o_mml);
   HIPCHK(hipMemcpyHtoDAsync(d, h.data(), total, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (p->var.tmode == 3) device_rows(c, p);
 }
 
 int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
@@ -626,10 +644,9 @@ const char* ftr_batch_error(const ftr_ctx* c, int i) {
 void ftr_problem_free(ftr_problem* p) {
   if (!p) return;
   /* the context may already be gone: only the device id is used */
-  if (p->d_slab) {
-    (void)hipSetDevice(p->device);
-    (void)hipFree(p->d_slab);
-  }
+  if (p->d_slab || p->d_rows_slab) (void)hipSetDevice(p->device);
+  if (p->d_slab) (void)hipFree(p->d_slab);
+  if (p->d_rows_slab) (void)hipFree(p->d_rows_slab);
   delete p;
 }
 
@@ -680,11 +697,7 @@ int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const f
     if (N > 0) {
       Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ, nullptr);
       ftr::Launch a;
-      a.prm = p->d_prm;
-      a.t_w = p->d_w;
-      a.t_jr = p->d_jr;
-      a.t_ir = p->d_ir;
-      a.t_mmL = p->d_mml;
+      fill_tables(a, p);
       a.block_state = G.d_states;
       a.lane_poly = G.d_polys;
       a.N = N;
@@ -753,11 +766,7 @@ int ftr_sample(ftr_ctx* c, ftr_problem* p, uint64_t first, uint64_t n, uint64_t 
     Geometry G = setup_stream(p, first, n, seed, 0, p->var.block, 2 + 4 * p->prep.K, nullptr);
     ftr::Launch a;
     memset(&a, 0, sizeof a);
-    a.prm = p->d_prm;
-    a.t_w = p->d_w;
-    a.t_jr = p->d_jr;
-    a.t_ir = p->d_ir;
-    a.t_mmL = p->d_mml;
+    fill_tables(a, p);
     a.block_state = G.d_states;
     a.lane_poly = G.d_polys;
     a.N = n;
@@ -895,11 +904,7 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
         ftr_problem* p = probs[i];
         ftr::BatchItem& it = items[g.item_off + t];
         memset(&it, 0, sizeof it);
-        it.a.prm = p->d_prm;
-        it.a.t_w = p->d_w;
-        it.a.t_jr = p->d_jr;
-        it.a.t_ir = p->d_ir;
-        it.a.t_mmL = p->d_mml;
+        fill_tables(it.a, p);
         it.a.block_state = d_states + 4 * state_off[i];
         it.a.lane_poly = d_polys + 4 * g.poly_off;
         it.a.N = Ns[i];
@@ -995,9 +1000,10 @@ int ftr_set_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block) {
   if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
   try {
     HIPCHK(hipSetDevice(c->device));
-    for (Variant v : candidate_variants(p->prep, c->prop.sharedMemPerBlock))
+    for (Variant v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c)))
       if (v.tmode == table_mode && v.block == block) {
         v.cparams = p->var.cparams;
+        if (v.tmode == 3) device_rows(c, p);
         p->m = load_module(c, full_source(p->prep, v));
         p->var = v;
         p->lds = variant_lds(v, p->prep.g.E);
@@ -1028,7 +1034,7 @@ int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, 
   Variant bestv = keep;
   /* (table placement, block) with the generic kernel ... */
   if (ftr_set_specialized(c, p, 0) != FTR_OK) return FTR_ERR_DEVICE;
-  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock)) {
+  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c))) {
     if (ftr_set_variant(c, p, v.tmode, v.block) != FTR_OK) continue;
     const double t = probe();
     if (t < best) {

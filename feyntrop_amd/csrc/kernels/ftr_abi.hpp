@@ -36,6 +36,14 @@ struct DevParams {
   double P[FTR_MAX_V * FTR_MAX_V];
 };
 
+/* Per-state record of the row-table placement (table mode 3). */
+struct alignas(16) StateRec {
+  double jr;      /* J(g) */
+  double ir;      /* 1/r(g) */
+  uint32_t mmL;   /* mm | L<<1 */
+  uint32_t pad[3];
+};
+
 /* Kernel arguments for one problem. */
 struct Launch {
   const DevParams* prm;
@@ -43,6 +51,8 @@ struct Launch {
   const double* t_jr;      /* J(g), 2^E */
   const double* t_ir;      /* 1/r(g), 2^E */
   const uint8_t* t_mmL;    /* mm | L<<1, 2^E */
+  const double* t_rows;    /* table mode 3: 2^E x E, row g holds w(g\e) for e in g, 0 else */
+  const StateRec* t_rec;   /* table mode 3: 2^E records */
   const uint64_t* block_state; /* nblocks x 4: stream state at the block's first sample */
   const uint64_t* lane_poly;   /* log2(blockDim) x 4: jump polynomials for S*(2E-1)*2^j */
   uint64_t N;              /* total samples */

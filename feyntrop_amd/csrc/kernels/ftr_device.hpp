@@ -40,9 +40,6 @@
 #endif
 #include "ftr_abi.hpp"
 
-#ifndef FTR_SCAN_PARALLEL
-#define FTR_SCAN_PARALLEL 0
-#endif
 
 namespace ftr {
 
@@ -183,39 +180,36 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
   for (int e = 0; e < E; e++) X[e] = 0.0;
   for (int j = 0; j < E; j++) {
     const double R = rng.uniform() * JJr;
-#if FTR_SCAN_PARALLEL
-    /* all candidate weights are loaded first (independent LDS reads), then
-     * the ascending running sum picks the first edge with T > R, or the last
-     * edge of the subgraph -- the same rule and summation order as the
-     * reference's visit_cuts */
-    double wv[E];
+    int e = 0;
+    if constexpr (TV::TM == 3) {
+      /* the state's row: all E weights loaded together, absent edges 0 */
+      const double* row = tv.row(g);
+      double rv[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) wv[q] = ((g >> q) & 1u) ? tv.w(g ^ (1u << q)) : 0.0;
-    double T = 0.0;
-    int e = 31 - __builtin_clz(g);
-    bool found = false;
+      for (int q = 0; q < E; q++) rv[q] = row[q];
+      double T = 0.0;
+      int last = 0;
+      bool found = false;
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-      if ((g >> q) & 1u) {
-        T += wv[q];
-        if (!found && T > R) {
-          found = true;
-          e = q;
-        }
+      for (int q = 0; q < E; q++) {
+        T += rv[q];
+        if ((g >> q) & 1u) last = q;
+        const bool hit = !found && T > R;
+        if (hit) e = q;
+        found = found || hit;
+      }
+      if (!found) e = last;
+    } else {
+      double T = 0.0;
+      uint32_t m = g;
+      while (m) {
+        const int q = __builtin_ctz(m);
+        m &= m - 1;
+        T += tv.w(g ^ (1u << q));
+        e = q;
+        if (T > R) break;
       }
     }
-#else
-    double T = 0.0;
-    int e = 0;
-    uint32_t m = g;
-    while (m) {
-      const int q = __builtin_ctz(m);
-      m &= m - 1;
-      T += tv.w(g ^ (1u << q));
-      e = q;
-      if (T > R) break;
-    }
-#endif
     const uint32_t c = g ^ (1u << e);
     JJr = tv.jr(c);
     const double ir = tv.ir(c);
@@ -608,9 +602,13 @@ struct ParamsFor<DevParams> {
 
 /* -------------------------------------------------------- table views -- */
 /* TMODE 0: whole table in LDS; 1: w = J/r in LDS, the rest from global
- * (L2-resident); 2: everything from global. */
+ * (L2-resident); 2: everything from global; 3: rows -- the E cut weights
+ * w(g\e) of a state contiguous in HBM plus a 32-byte record per state, so a
+ * cut step reads one row and one record instead of |g| scattered entries
+ * (large tables). */
 template <int TMODE>
 struct TableView {
+  static constexpr int TM = TMODE;
   const double* pw;
   const double* pjr;
   const double* pir;
@@ -619,6 +617,19 @@ struct TableView {
   __device__ inline double jr(uint32_t g) const { return pjr[g]; }
   __device__ inline double ir(uint32_t g) const { return pir[g]; }
   __device__ inline uint32_t mmL(uint32_t g) const { return pmm[g]; }
+};
+template <>
+struct TableView<3> {
+  static constexpr int TM = 3;
+  const double* pw;
+  const double* rows;
+  const StateRec* rec;
+  int E;
+  __device__ inline double w(uint32_t g) const { return pw[g]; }
+  __device__ inline double jr(uint32_t g) const { return rec[g].jr; }
+  __device__ inline double ir(uint32_t g) const { return rec[g].ir; }
+  __device__ inline uint32_t mmL(uint32_t g) const { return rec[g].mmL; }
+  __device__ inline const double* row(uint32_t g) const { return rows + (size_t)g * E; }
 };
 
 template <int TMODE>
@@ -631,7 +642,10 @@ template <int TMODE>
 __device__ inline TableView<TMODE> stage_table(const Launch& a, int E, char* lds) {
   TableView<TMODE> tv;
   const uint32_t NT = 1u << E;
-  if constexpr (TMODE == 2) {
+  if constexpr (TMODE == 3) {
+    tv.pw = a.t_w; tv.rows = a.t_rows; tv.rec = a.t_rec; tv.E = E;
+    return tv;
+  } else if constexpr (TMODE == 2) {
     tv.pw = a.t_w; tv.pjr = a.t_jr; tv.pir = a.t_ir; tv.pmm = a.t_mmL;
     return tv;
   } else {

@@ -273,3 +273,20 @@ extern "C" __global__ void ftrj_sampler_tables(uint64_t NT, const double* Jr, co
   ir[i] = 1. / rr[i];
   mml[i] = (uint8_t)((mmv[i] ? 1 : 0) | (Lt[i] << 1));
 }
+
+/* Row-table placement (sampler table mode 3): rows[g E + e] = w(g\e) for e in
+ * g, else 0; rec[g] = {J(g), 1/r(g), mm | L<<1 (as the low 32 bits of the
+ * third double), 0}. */
+extern "C" __global__ void ftrj_rows(uint64_t NT, int E, const double* w, const double* Jr, const double* ir,
+                                     const uint8_t* mml, double* rows, double* rec) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= NT) return;
+  for (int e = 0; e < E; e++) rows[g * E + e] = ((g >> e) & 1) ? w[g ^ ((uint64_t)1 << e)] : 0.0;
+  double* r = rec + g * 4;
+  r[0] = Jr[g];
+  r[1] = ir[g];
+  uint32_t* m = reinterpret_cast<uint32_t*>(r + 2);
+  m[0] = mml[g];
+  m[1] = 0;
+  r[3] = 0.0;
+}

@@ -162,6 +162,33 @@ def test_estimate_equals_reference_one_thread(engine, golden_problems, name):
             assert abs(s2[k, q] - rs2) <= 1e-9 * abs(rs2) + 1e-300, (k, q)
 
 
+@pytest.mark.parametrize("name", ["triple_ladder", "cubic5"])
+def test_row_tables_equal_reference_one_thread(engine, golden_problems, name):
+    """Table mode 3 (per-state rows of cut weights in HBM) draws the same
+    samples as the other placements: the reference's one-thread moments;
+    sampler points equal the golden ones."""
+    print(f"[rows] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    engine.set_variant(p, 3, 256)
+    assert engine.variant(p) == (3, 256)
+    g = golden_problems[name]["estimate"]
+    N = g["N"]
+    ref = _hex(g["moments"]).reshape(p.K, 2, 3)
+    m = engine.integrate(p, N, 0)
+    n, s1, s2 = m.arrays()
+    for k in range(p.K):
+        assert n[k, 0] == ref[k, 0, 0]
+        for q in ((0, 1) if p.info.path == 2 else (0,)):
+            rn, rs1, rs2 = ref[k, q]
+            sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+            assert abs(s1[k, q] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn), (k, q)
+    gs = golden_problems[name]["samples"]
+    Xr = _hex(gs["X"]).reshape(-1, p.E)
+    X, _ = gpu_sample(engine, p, 0, Xr.shape[0])
+    assert np.array_equal(np.argsort(-X, axis=1, kind="stable"), np.argsort(-Xr, axis=1, kind="stable"))
+    assert np.max(np.abs(X - Xr) / np.abs(Xr)) <= 1e-12
+
+
 @pytest.mark.parametrize("name", ["wheel_k4", "double_box", "triple_ladder", "2L_3pt_tutorial"])
 def test_specialized_kernel_equals_reference_one_thread(engine, golden_problems, name):
     """The kernel specialised to the kinematic point (constants folded at
