@@ -124,6 +124,50 @@ def run_scan(eng, n_points, seed=0):
             "dropped": int(sum(m.dropped for m in outs))}
 
 
+def run_large(eng, name="ladder7", steps=2):
+    """A 22-edge planar ladder: the 2^22-entry J-table built on the device
+    (timed against the host pass), then plain sampling with the tables read
+    from L2/HBM."""
+    from feyntrop_amd import workloads as W
+    prob = W.LARGE[name]()
+    out = {}
+    for mode, tag in ((2, "device"), (1, "host")):
+        eng.set_jtable_mode(mode)
+        t0 = time.perf_counter()
+        p = eng.prepare(prob)
+        out[f"prepare_s_{tag}_jtable"] = time.perf_counter() - t0
+    eng.set_jtable_mode(0)
+    N = int(prob["N"])
+    eng.integrate(p, 1_000_000, 1)
+    t0 = time.perf_counter()
+    ms = [eng.integrate(p, N, 100 + i).kernel_ms for i in range(steps)]
+    wall = time.perf_counter() - t0
+    fps = flops_per_sample(p.info)
+    out.update({"workload": f"{name}: E={p.E} V={p.V} K={p.K}, 2^{p.E} J-table entries",
+                "value": N * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
+                "roofline_frac": fps * N / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS})
+    return out
+
+
+def run_tuned(eng, name, steps=2):
+    """A named workload with the autotuner's kernel variant."""
+    from feyntrop_amd import workloads as W
+    prob = W.all_problems()[name]()
+    p = eng.prepare(prob)
+    tm, blk, _ = eng.autotune(p, n_probe=min(int(prob["N"]), 20_000_000))
+    eng.integrate(p, prob["N"], 1)
+    t0 = time.perf_counter()
+    ms = []
+    for i in range(steps):
+        ms.append(eng.integrate(p, prob["N"], 100 + i).kernel_ms)
+    wall = time.perf_counter() - t0
+    fps = flops_per_sample(p.info)
+    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}", "N_per_step": prob["N"],
+            "value": prob["N"] * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
+            "variant": {"table_mode": tm, "block": blk},
+            "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+
+
 def _rel_err(p, m):
     r = p.finalize(m)
     I0 = complex(r[0, 0, 0], r[0, 1, 0])
@@ -300,6 +344,8 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ns = args.cpu_samples or 2.5e6 * threads
         line["cpu_baseline"] = cpu_baseline(prob, ns, threads)
+        # the same reference binary on one core (bounded: 2e6 samples)
+        line["cpu_baseline_1core"] = cpu_baseline(prob, 2e6, 1)
     print(json.dumps(line), flush=True)
     if args.calib_copy_mib:
         calibration_copy(args.calib_copy_mib << 20)

@@ -97,6 +97,7 @@ def lib():
     L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_set_jtable_mode.argtypes = [vp, C.c_int]
+    L.ftr_batch_graph_replays.argtypes = [vp]
     L.ftr_prepare_batch.argtypes = [vp, C.c_int, C.POINTER(ftr_input), C.POINTER(vp), C.POINTER(C.c_int)]
     L.ftr_batch_error.restype = C.c_char_p
     L.ftr_batch_error.argtypes = [vp, C.c_int]
@@ -254,6 +255,9 @@ class Engine:
         if rc:
             raise self._err(rc)
         return Problem(self, out, prob)
+
+    def batch_graph_replays(self):
+        return lib().ftr_batch_graph_replays(self.h)
 
     def prepare_batch(self, probs):
         """Prepare many problems (host phases in parallel).  Returns a list with

@@ -48,6 +48,9 @@ struct ftr_ctx {
   size_t pinned_bytes = 0;
   int jtable_mode = 0;        /* 0 auto, 1 host, 2 device (ftr_set_jtable_mode) */
   std::vector<std::string> batch_errors; /* messages of ftr_prepare_batch */
+  hipGraphExec_t batch_exec = nullptr;   /* last batch graph, replayed for equal shapes */
+  std::string batch_key;
+  int batch_replays = 0;
 };
 
 struct ftr_problem {

@@ -447,6 +447,7 @@ void ftr_destroy(ftr_ctx* c) {
   (void)hipSetDevice(c->device);
   for (auto& kv : c->raw_modules) (void)hipModuleUnload(kv.second);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
   if (c->d_batch) (void)hipFree(c->d_batch);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -874,6 +875,8 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
                  b_polys = al(n_polys * 32 + 32), b_part = al(n_part * 8 + 8);
     const size_t need = b_items + b_states + b_polys + b_part;
     if (need > c->batch_bytes) {
+      if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
+      c->batch_exec = nullptr;
       if (c->d_batch) HIPCHK(hipFree(c->d_batch));
       c->d_batch = nullptr;
       HIPCHK(hipMalloc(&c->d_batch, need));
@@ -932,33 +935,50 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
       HIPCHK(hipMemcpyHtoDAsync(d_polys, polys.data(), polys.size() * 8, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       if (n_part * 8 > c->pinned_bytes) {
+        if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
+        c->batch_exec = nullptr;
         if (c->h_pinned) HIPCHK(hipHostFree(c->h_pinned));
         c->h_pinned = nullptr;
         HIPCHK(hipHostMalloc((void**)&c->h_pinned, n_part * 8, hipHostMallocDefault));
         c->pinned_bytes = n_part * 8;
       }
       double* h_part = c->h_pinned;
-      /* one graph: all group kernels, then the copy-back */
-      hipGraph_t graph;
-      hipGraphExec_t exec;
-      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      for (auto& g : gs) {
-        const ftr::BatchItem* it = d_items + g.item_off;
-        int ni = (int)g.idx.size();
-        void* args[] = {(void*)&it, &ni};
-        HIPCHK(hipModuleLaunchKernel(g.f, g.nblocks, 1, 1, g.block, 1, 1, (unsigned)g.lds, c->stream, args, nullptr));
+      /* one graph: all group kernels, then the copy-back.  Its nodes depend
+       * only on the launch shapes and buffer addresses (the items, states
+       * and polynomials are uploaded above, outside the graph), so a call
+       * with the same shapes replays the previous graph */
+      std::ostringstream key;
+      key << (const void*)c->d_batch << ' ' << (const void*)h_part << ' ' << n_part;
+      for (auto& g : gs)
+        key << ' ' << (const void*)g.f << ':' << g.nblocks << ':' << g.block << ':' << g.lds << ':' << g.item_off << ':'
+            << g.idx.size();
+      if (!c->batch_exec || c->batch_key != key.str()) {
+        if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
+        c->batch_exec = nullptr;
+        c->batch_key.clear();
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (auto& g : gs) {
+          const ftr::BatchItem* it = d_items + g.item_off;
+          int ni = (int)g.idx.size();
+          void* args[] = {(void*)&it, &ni};
+          HIPCHK(hipModuleLaunchKernel(g.f, g.nblocks, 1, 1, g.block, 1, 1, (unsigned)g.lds, c->stream, args, nullptr));
+        }
+        HIPCHK(hipMemcpyDtoHAsync(h_part, d_part, n_part * 8, c->stream));
+        HIPCHK(hipStreamEndCapture(c->stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&c->batch_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+        c->batch_key = key.str();
+      } else {
+        c->batch_replays++;
       }
-      HIPCHK(hipMemcpyDtoHAsync(h_part, d_part, n_part * 8, c->stream));
-      HIPCHK(hipStreamEndCapture(c->stream, &graph));
-      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       HIPCHK(hipEventRecord(c->ev0, c->stream));
-      HIPCHK(hipGraphLaunch(exec, c->stream));
+      HIPCHK(hipGraphLaunch(c->batch_exec, c->stream));
       HIPCHK(hipEventRecord(c->ev1, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
       for (auto& g : gs)
         for (int i : g.idx) {
           const int K = probs[i]->prep.K, NQ = 2 + 4 * K;
@@ -1067,6 +1087,8 @@ int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, 
   if (best_ms) *best_ms = best;
   return FTR_OK;
 }
+
+int ftr_batch_graph_replays(const ftr_ctx* c) { return c ? c->batch_replays : 0; }
 
 int ftr_get_specialized(const ftr_problem* p, int* on) {
   if (!p || !on) return FTR_ERR_INVALID;

@@ -135,9 +135,14 @@ int ftr_replay(ftr_ctx* ctx, ftr_problem* p, uint64_t n, const double* X_dev,
 int ftr_run_json(ftr_ctx* ctx, const char* json_in, char** json_out, char** log_out);
 void ftr_free_string(char* s);
 
-/* Batched scan: n problems, one launch of a persistent kernel per batch. */
+/* Batched scan (no reference counterpart): n problems in one hipGraph (one
+ * multi-problem kernel per group of problems sharing a kernel, then the
+ * copy-back).  A call with the same launch shapes as the previous one (same
+ * problems and sample counts; seeds may differ) replays that graph. */
 int ftr_integrate_batch(ftr_ctx* ctx, int n, ftr_problem* const* probs, const uint64_t* N,
                         const uint64_t* seeds, ftr_moments* out);
+/* Number of batch calls served by replaying the previous graph. */
+int ftr_batch_graph_replays(const ftr_ctx* ctx);
 
 /* Autotuner (no reference counterpart; replaces the reference's fixed OpenMP
  * chunking): times every feasible (J-table placement, block size) kernel

@@ -327,6 +327,28 @@ def test_prepare_batch_equals_individual(engine):
     assert n_ok > 40
 
 
+def test_batch_graph_replays_match_eager(engine):
+    """Three batch calls with the same shapes and different seeds replay one
+    captured graph (with eager integrations and synchronisations between
+    the replays); every replay equals the one-by-one integrations."""
+    import feyntrop_amd
+    pts = W.scan_points(12, seed=9, accept=feyntrop_amd.host_accepts)
+    probs = [engine.prepare(p) for p in pts]
+    Ns = [min(p["N"], 200_000) for p in pts]
+    r0 = engine.batch_graph_replays()
+    for rep in range(3):
+        seeds = [p["seed"] + 1000 * rep for p in pts]
+        outs = engine.integrate_batch(probs, Ns, seeds)
+        for q, N, s, mb in zip(probs, Ns, seeds, outs):
+            mi = engine.integrate(q, N, s)
+            nb, s1b, s2b = mb.arrays()
+            ni, s1i, s2i = mi.arrays()
+            assert np.array_equal(nb, ni) and mb.dropped == mi.dropped
+            assert np.allclose(s1b, s1i, rtol=1e-10, atol=1e-12 * np.abs(s1i).max())
+            assert np.allclose(s2b, s2i, rtol=1e-10, atol=0)
+    assert engine.batch_graph_replays() - r0 == 2
+
+
 def test_batch_equals_individual(engine):
     """The batched scan draws the same samples as one call per problem: the
     same counts, sums equal up to summation order."""
