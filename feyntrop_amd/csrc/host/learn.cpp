@@ -289,6 +289,7 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->mlp = load_mlp(c);
     l->var = p->var;
     l->var.block = LEARN_BLOCK;
+    l->var.cparams = 0; /* the learned kernels read the parameters from memory */
     l->lds = variant_lds(l->var, E);
     const hipModule_t gm = load_module_raw(c, full_source(p->prep, l->var, true));
     HIPCHK(hipModuleGetFunction(&l->f_lev, gm, "ftrl_lev"));

@@ -70,7 +70,11 @@ Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit)
   const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
   const size_t cap0 = std::min<size_t>(lds_limit, 64 * 1024);
   v.block = 256;
-  if (full <= cap0) {
+  if (pr.path == 2 && rows_bytes(E) <= mem_limit) {
+    /* deformed path: the row layout (all E weights of a state in flight at
+     * once) beats the LDS scans (measured +5..9 % on 5- to 10-edge graphs) */
+    v.tmode = 3;
+  } else if (full <= cap0) {
     v.tmode = 0;
   } else if (wonly <= lds_limit) {
     /* one table copy fills most of the LDS, so one block per CU: make it a
@@ -96,7 +100,7 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
       const size_t t = tm == 0 ? full : (tm == 1 ? wonly : 0);
       if (t + (size_t)(blk / 64) * (2 + 4 * pr.K) * 8 + 16 > lds_limit) continue;
       if (tm == 0 && full > 64 * 1024 && blk == 256) continue;
-      if (tm == 3 && (E < 10 || rows_bytes(E) > mem_limit)) continue;
+      if (tm == 3 && rows_bytes(E) > mem_limit) continue;
       Variant v;
       v.K = pr.K;
       v.mode = pr.path;

@@ -15,6 +15,7 @@ from feyntrop_amd import host_path, precompile, precompile_learned, precompile_s
 def problems():
     probs = dict(workloads.all_problems())
     probs["ladder7"] = workloads.LARGE["ladder7"]   # the large-graph bench line
+    probs["ladder5"] = workloads.LARGE["ladder5"]   # learned proposal on row tables (tests)
     for k, v in workloads.scan_problems(host_path).items():
         probs[k] = v
     return probs
@@ -35,7 +36,7 @@ def _one(item):
 
 
 def main():
-    named = set(workloads.all_problems())
+    named = set(workloads.all_problems()) | {"ladder5"}
     # learned-proposal kernels for the named workloads only
     items = [(k, v, k in named) for k, v in problems().items()]
     workers = min(8, os.cpu_count() or 1)

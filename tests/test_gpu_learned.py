@@ -113,9 +113,15 @@ def test_tables_and_sampler_replay(engine, oracle, name):
     np.testing.assert_allclose(wt[:4000], LR.weight_from_path(X[:4000], order[:4000], t, w, Jr, r), rtol=1e-9)
 
 
-@pytest.mark.parametrize("name", ["double_box", "triple_ladder", "2L_3pt_tutorial"])
+@pytest.mark.parametrize("name", ["double_box", "triple_ladder", "2L_3pt_tutorial", "ladder5"])
 def test_untrained_learner_is_plain_sampler(engine, name):
-    prob, p = _setup(engine, name)
+    """Zero output layer: the plain sampler's samples and estimate; ladder5
+    (16 edges) runs with the row-table placement, the others from LDS."""
+    if name in W.LARGE:
+        prob = W.LARGE[name]()
+        p = engine.prepare(prob)
+    else:
+        prob, p = _setup(engine, name)
     lr = feyntrop_amd.Learner(engine, p, init_seed=1)
     N = 2_000_000
     m0 = engine.integrate(p, N, 4)
