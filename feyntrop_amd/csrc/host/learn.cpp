@@ -91,6 +91,7 @@ struct ftr_learner {
   double *d_lw = nullptr, *d_lZ = nullptr, *d_lratio = nullptr, *d_lia = nullptr, *d_lrad = nullptr,
          *d_ldiff = nullptr, *d_r = nullptr, *d_gC = nullptr, *d_gS = nullptr, *d_gsum = nullptr;
   ftr::LQuad* d_lst = nullptr; /* per-state records read by the sampler */
+  double* d_ltw = nullptr;      /* per-cut ratio factors w(g\e)/lw(g,e) */
   int adam_t = 0;
   bool tables_valid = false;
   bool use_plain = false; /* set by ftr_learner_select when the plain sampler wins */
@@ -198,8 +199,9 @@ void backward(ftr_ctx* c, ftr_learner* l) {
 void build_tables(ftr_ctx* c, ftr_learner* l) {
   int E = l->E;
   double dmax = DMAX, smax = SMAX;
-  void* args[] = {&l->d_Y, &E, &l->p->d_w, &l->p->d_jr, &l->d_r, &l->d_lw, &l->d_lZ, &l->d_lratio,
-                  &l->d_lia, &l->d_lrad, &l->d_ldiff, &dmax, &smax, &l->d_lst};
+  void* args[] = {&l->d_Y,    &E,          &l->p->d_w, &l->p->d_jr, &l->d_r,    &l->d_lw,          &l->d_lZ,
+                  &l->d_lratio, &l->d_lia, &l->d_lrad, &l->d_ldiff, &dmax, &smax, &l->d_lst, &l->p->d_mml,
+                  &l->d_ltw};
   launch(l->mlp.tables, (l->NT + 255) / 256, 1, 256, c->stream, args);
 }
 
@@ -221,6 +223,7 @@ ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
   L.a.S = G.S;
   L.a.partials = G.d_partials;
   L.lst = l->d_lst;
+  L.ltw = l->d_ltw;
   L.lw = l->d_lw;
   L.lZ = l->d_lZ;
   L.lratio = l->d_lratio;
@@ -337,6 +340,7 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->d_gS = dalloc<double>(l, NT);
     l->d_gsum = dalloc<double>(l, 2);
     l->d_lst = dalloc<ftr::LQuad>(l, NT);
+    l->d_ltw = dalloc<double>(l, NT * E);
     /* initial parameters: hidden layers uniform(+-sqrt(3/fan_in)), zero
      * biases, zero output layer (q = p_trop) */
     std::vector<float> par(NPAR, 0.0f);

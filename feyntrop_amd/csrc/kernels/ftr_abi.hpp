@@ -79,14 +79,18 @@ struct BatchItem {
  * cut e; gS[g] += c (1 + a(g) log Y) for the radius drawn after arriving at
  * g; gsum[0] += c, gsum[1] += 1, with c = weight * |R_0|. */
 /* Per-state record of the learned tables, one 64-byte load per visited state:
- * lZ(g), A(g) = lZ(g)/J(g) and the radial parameters lia, lrad, ldiff. */
+ * lZ(g), A(g) = lZ(g)/J(g), the radial parameters lia, lrad, ldiff and the
+ * state's mm | L<<1 flags. */
 struct alignas(64) LQuad {
-  double lZ, A, lia, lrad, ldiff, pad0, pad1, pad2;
+  double lZ, A, lia, lrad, ldiff;
+  uint32_t mmL, pad0;
+  double pad1, pad2;
 };
 
 struct LearnLaunch {
   Launch a;
   const LQuad* lst;
+  const double* ltw;   /* 2^E x E: w(g\e) / lw(g, e), the per-cut ratio factor */
   const double* lw;
   const double* lZ;
   const double* lratio;

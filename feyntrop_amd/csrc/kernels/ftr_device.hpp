@@ -804,27 +804,31 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
  * not one per candidate), and the running sum over them in ascending edge
  * order selects the cut as the tropical sampler does (zero weights of
  * absent edges leave the sum unchanged). */
-template <class G, class TV, bool KEEP_LY>
-__device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV& tv, double (&X)[G::E],
-                                   double& lpsi, double& lG, double& wgt, uint64_t& path, double* ly_out,
-                                   uint64_t ly_stride) {
+template <class G, bool KEEP_LY>
+__device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&X)[G::E], double& lpsi,
+                                   double& lG, double& wgt, uint64_t& path, double* ly_out, uint64_t ly_stride) {
   constexpr int E = G::E;
   uint32_t g = (1u << E) - 1u;
   LQuad st = L.lst[g];
-  const uint32_t b0 = tv.mmL(g);
-  bool MM = b0 & 1;
-  int oL = (int)(b0 >> 1);
+  bool MM = st.mmL & 1;
+  int oL = (int)(st.mmL >> 1);
   double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
   path = 0;
 #pragma unroll
   for (int e = 0; e < E; e++) X[e] = 0.0;
   for (int j = 0; j < E; j++) {
     const double R = rng.uniform() * st.lZ;
+    /* the state's learned weights and plain/learned weight ratios, all in
+     * flight together */
     const double* row = L.lw + (size_t)g * E;
-    double rv[E];
+    const double* trow = L.ltw + (size_t)g * E;
+    double rv[E], tv[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) rv[q] = row[q];
-    double T = 0.0, lsel = 0.0, lastv = 0.0;
+    for (int q = 0; q < E; q++) {
+      rv[q] = row[q];
+      tv[q] = trow[q];
+    }
+    double T = 0.0, tsel = 0.0, lastt = 0.0;
     int e = 0, last = 0;
     bool found = false;
 #pragma unroll
@@ -832,25 +836,24 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, const TV&
       T += rv[q];
       if ((g >> q) & 1u) {
         last = q;
-        lastv = rv[q];
+        lastt = tv[q];
       }
       const bool hit = !found && T > R;
       if (hit) {
         e = q;
-        lsel = rv[q];
+        tsel = tv[q];
       }
       found = found || hit;
     }
     if (!found) {
       e = last;
-      lsel = lastv;
+      tsel = lastt;
     }
     const uint32_t c = g ^ (1u << e);
-    ratio *= st.A * (tv.w(c) / lsel);
+    ratio *= st.A * tsel;
     st = L.lst[c];
-    const uint32_t b = tv.mmL(c);
-    const bool mm = b & 1;
-    const int Lc = (int)(b >> 1);
+    const bool mm = st.mmL & 1;
+    const int Lc = (int)(st.mmL >> 1);
     if (MM && !mm) lGv = lX;
     MM = mm;
 #pragma unroll
@@ -888,7 +891,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   constexpr int NQ = 2 + 4 * K;
   const Launch& a = L.a;
   const DevParams& p = *a.prm;
-  TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
+  /* the learned tables are read from L2/HBM: the LDS holds the reduction */
   double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
   Xoshiro rng = lane_stream(a, blockIdx.x);
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -900,8 +903,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, TableView<TMODE>, GRAD>(L, rng, tv, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr,
-                                           a.N);
+    learn_point<G, GRAD>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N);
     if (GRAD) L.gpath[i] = path;
     cplx out[K];
     if (!integrand<G, K, MODE>(p, X, log_scale(p, lF, lGv), out)) {
@@ -1111,7 +1113,6 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   extern __shared__ __align__(16) char lds[];
   constexpr int E = G::E;
   const Launch& a = L.a;
-  TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
   Xoshiro rng = lane_stream(a, blockIdx.x);
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
@@ -1119,7 +1120,7 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, TableView<TMODE>, false>(L, rng, tv, X, lF, lGv, wgt, path, nullptr, 0);
+    learn_point<G, false>(L, rng, X, lF, lGv, wgt, path, nullptr, 0);
 #pragma unroll
     for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
     wout[i] = wgt;

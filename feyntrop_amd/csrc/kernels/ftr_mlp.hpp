@@ -14,6 +14,7 @@
 #pragma once
 
 #ifdef __HIPCC_RTC__
+typedef __hip_internal::uint8_t uint8_t;
 typedef __hip_internal::uint32_t uint32_t;
 typedef __hip_internal::uint64_t uint64_t;
 #else
@@ -172,7 +173,8 @@ extern "C" __global__ void mlp_to_bf16(const float* src, unsigned short* dst, ui
  * Delta = dmax tanh(y/dmax), sigma = smax tanh(y/smax). */
 extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double* w, const double* jr,
                                             const double* r, double* lw, double* lZ, double* lratio, double* lia,
-                                            double* lrad, double* ldiff, double dmax, double smax, double* lst) {
+                                            double* lrad, double* ldiff, double dmax, double smax, double* lst,
+                                            const uint8_t* mml, double* ltw) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (1u << E)) return;
   const float* y = Y + (size_t)g * 32;
@@ -186,7 +188,9 @@ extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double*
   lZ[g] = Z;
   for (int e = 0; e < E; e++) {
     const double v = lw[(size_t)g * E + e];
-    lratio[(size_t)g * E + e] = ((g >> e) & 1u) ? (Z * w[g ^ (1u << e)]) / (jr[g] * v) : 0.0;
+    const bool in = (g >> e) & 1u;
+    lratio[(size_t)g * E + e] = in ? (Z * w[g ^ (1u << e)]) / (jr[g] * v) : 0.0;
+    ltw[(size_t)g * E + e] = in ? w[g ^ (1u << e)] / v : 0.0;
   }
   const double sg = smax * tanh((double)y[E] / smax);
   const double aa = r[g] * exp(sg);
@@ -200,7 +204,10 @@ extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double*
   q[2] = 1.0 / aa;
   q[3] = -sg;
   q[4] = r[g] - aa;
-  q[5] = q[6] = q[7] = 0.0;
+  uint32_t* qi = reinterpret_cast<uint32_t*>(q + 5);
+  qi[0] = mml[g];
+  qi[1] = 0;
+  q[6] = q[7] = 0.0;
 }
 
 /* dL/dY of the forward KL  L = -sum_paths c log q(path) / sum c  from the
