@@ -818,39 +818,25 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
   for (int e = 0; e < E; e++) X[e] = 0.0;
   for (int j = 0; j < E; j++) {
     const double R = rng.uniform() * st.lZ;
-    /* the state's learned weights and plain/learned weight ratios, all in
-     * flight together */
+    /* the state's learned weights, all in flight together */
     const double* row = L.lw + (size_t)g * E;
-    const double* trow = L.ltw + (size_t)g * E;
-    double rv[E], tv[E];
+    double rv[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-      rv[q] = row[q];
-      tv[q] = trow[q];
-    }
-    double T = 0.0, tsel = 0.0, lastt = 0.0;
+    for (int q = 0; q < E; q++) rv[q] = row[q];
+    double T = 0.0;
     int e = 0, last = 0;
     bool found = false;
 #pragma unroll
     for (int q = 0; q < E; q++) {
       T += rv[q];
-      if ((g >> q) & 1u) {
-        last = q;
-        lastt = tv[q];
-      }
+      if ((g >> q) & 1u) last = q;
       const bool hit = !found && T > R;
-      if (hit) {
-        e = q;
-        tsel = tv[q];
-      }
+      if (hit) e = q;
       found = found || hit;
     }
-    if (!found) {
-      e = last;
-      tsel = lastt;
-    }
+    if (!found) e = last;
     const uint32_t c = g ^ (1u << e);
-    ratio *= st.A * tsel;
+    ratio *= st.A * L.ltw[(size_t)g * E + e];
     st = L.lst[c];
     const bool mm = st.mmL & 1;
     const int Lc = (int)(st.mmL >> 1);
