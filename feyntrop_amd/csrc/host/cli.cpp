@@ -6,9 +6,12 @@
  *   {"IGtr":..,"integral":[[[re,dre],[im,dim]],..],"seconds preprocessing":..,
  *    "seconds sampling":..}
  * (:264-278), keys sorted, numbers formatted like the reference's writer.
- * Addition: {"batch": [problem, ...]} integrates many problems in one
- * batched launch and answers {"batch": [result, ...]}.
+ * Additions: {"batch": [problem, ...]} integrates many problems in one
+ * batched launch and answers {"batch": [result, ...]}; "proposal": "learned"
+ * in a problem integrates with the learned proposal (E <= 16).
  */
+#include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <sstream>
@@ -198,12 +201,46 @@ extern "C" int ftr_run_json(ftr_ctx* ctx, const char* json_in, char** json_out, 
       if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
       /* regime / GP log lines were collected during preprocessing */
       for (const std::string& l : ftr_problem_log(prob)) log += l + "\n";
+      /* optional addition: "proposal": "learned" trains the learned proposal
+       * first (the plain sampler judges it); absent or "plain" keeps the
+       * reference's sampler */
+      std::string proposal = "plain";
+      if (const Json* pj = data.get("proposal")) {
+        if (pj->t != Json::Str || (pj->s != "plain" && pj->s != "learned"))
+          throw Error(FTR_ERR_INVALID, "input proposal must be \"plain\" or \"learned\"");
+        proposal = pj->s;
+      }
       log += "Started integrating on 1 GPU and N = " + g_str((double)c.N) + " points.\n";
       ftr_moments m;
       if (!ctx) throw Error(FTR_ERR_DEVICE, "feyntrop: no usable HIP device");
-      rc = ftr_integrate(ctx, prob, c.N, c.seed, nullptr, &m);
-      if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
-      const std::string o = format_result(prob, m, m.seconds, log);
+      double t_learn = 0.0;
+      if (proposal == "learned") {
+        const auto t0 = std::chrono::steady_clock::now();
+        ftr_learner* l = nullptr;
+        rc = ftr_learner_create(ctx, prob, 1, &l);
+        if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
+        struct Guard {
+          ftr_learner* l;
+          ~Guard() { ftr_learner_free(l); }
+        } guard{l};
+        rc = ftr_learner_train(ctx, l, 400, 1u << 18, c.seed + 100, 3e-3, nullptr);
+        if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
+        int use = 0;
+        double ep = 0, el = 0;
+        const uint64_t probe = std::min<uint64_t>(4000000, std::max<uint64_t>(c.N / 10, 100000));
+        rc = ftr_learner_select(ctx, l, probe, c.seed + 7, &use, &ep, &el);
+        if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
+        t_learn = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        log += std::string("Learned proposal: ") + (use ? "used" : "not better than the plain sampler, not used") +
+               " (relative variance x time per sample: plain " + g_str(ep) + ", learned " + g_str(el) +
+               "; training " + g_str(t_learn) + " s).\n";
+        rc = ftr_learner_integrate(ctx, l, c.N, c.seed, nullptr, &m);
+        if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
+      } else {
+        rc = ftr_integrate(ctx, prob, c.N, c.seed, nullptr, &m);
+        if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
+      }
+      const std::string o = format_result(prob, m, m.seconds + t_learn, log);
       if (json_out) *json_out = dupstr(o);
     }
     rc = FTR_OK;

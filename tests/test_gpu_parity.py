@@ -279,6 +279,27 @@ def test_cli_readme_wheel(golden_fixtures):
     assert "(Effective) kinematic regime: Euclidean (generic)." in out.stderr
 
 
+def test_cli_learned_proposal(golden_fixtures):
+    """"proposal": "learned" on the tutorial graph: same output format, every
+    eps coefficient within 3 combined standard errors of tests/*.txt, and a
+    log line that names the proposal used."""
+    import feyntrop_amd
+    prob = W.fixture_2L_3pt_tutorial(N=10**7)
+    prob["proposal"] = "learned"
+    out = subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps(prob), capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout)
+    assert list(d.keys()) == ["IGtr", "integral", "seconds preprocessing", "seconds sampling"]
+    assert "Learned proposal: used" in out.stderr
+    gold = golden_fixtures["2L_3pt_tutorial"]
+    for k in range(5):
+        for q in range(2):
+            a, da = d["integral"][k][q]
+            b, db = gold[k][q]
+            assert abs(a - b) <= 3 * np.hypot(da, db), (k, q, a, da, b, db)
+
+
 def test_cli_batch_mode_equals_single_runs():
     """{"batch": [...]}: one result per problem, each the single-problem run
     of the binary: the same samples, so IGtr identical and every value and

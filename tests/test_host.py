@@ -90,6 +90,14 @@ def test_cli_rejects_bad_input(label, mut, msg):
     assert msg in r.stderr
 
 
+def test_cli_rejects_unknown_proposal():
+    p = W.wheel_k4(N=1000)
+    p["proposal"] = "magic"
+    r = _cli(p)
+    assert r.returncode != 0 and r.stdout == ""
+    assert 'input proposal must be "plain" or "learned"' in r.stderr
+
+
 def test_cli_batch_rejects_bad_item():
     """Batch mode names the failing item and keeps the item's own message."""
     good = W.wheel_k4(N=1000)
