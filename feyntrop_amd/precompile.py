@@ -16,6 +16,7 @@ def problems():
     probs = dict(workloads.all_problems())
     probs["ladder7"] = workloads.LARGE["ladder7"]   # the large-graph bench line
     probs["ladder5"] = workloads.LARGE["ladder5"]   # learned proposal on row tables (tests)
+    probs["ladder4_mink"] = workloads.LARGE["ladder4_mink"]   # large deformed graph (tests)
     for k, v in workloads.scan_problems(host_path).items():
         probs[k] = v
     return probs

@@ -191,18 +191,24 @@ FIXTURES = {
     "2L_5pt_box_pent": fixture_2L_5pt_box_pent,
 }
 
-def ladder_4pt(rungs=6, N=10**7, seed=0, D=4, mass=1.0):
+def ladder_4pt(rungs=6, N=10**7, seed=0, D=4, mass=1.0, minkowski=False, lam=0.5):
     """Planar 4-point ladder with `rungs` loops (V = 2 (rungs + 1), E =
-    3 rungs + 1), Euclidean kinematics s = -3, t = -2 on the corner
-    vertices, all masses equal: a large J-table (2^E entries)."""
+    3 rungs + 1), all masses equal: a large J-table (2^E entries).
+    Euclidean kinematics s = -3, t = -2 on the corner vertices (real path),
+    or with minkowski=True the double box's s, t with p_i^2 = 1 and the
+    deformation lambda (complex path)."""
     top = list(range(rungs + 1))
     bot = [rungs + 1 + i for i in range(rungs + 1)]
     pairs = [(top[i], bot[i]) for i in range(rungs + 1)]
     pairs += [(top[i], top[i + 1]) for i in range(rungs)] + [(bot[i], bot[i + 1]) for i in range(rungs)]
     V = 2 * (rungs + 1)
-    P = _four_point(V, (top[0], bot[0], top[-1], bot[-1]), -3.0, -2.0, -1.0)
+    if minkowski:
+        s_, t_ = _st_from_seed(0)
+        P = _four_point(V, (top[0], bot[0], top[-1], bot[-1]), s_, t_, 1.0)
+    else:
+        P = _four_point(V, (top[0], bot[0], top[-1], bot[-1]), -3.0, -2.0, -1.0)
     return {"graph": _edges(pairs), "dimension": D, "scalarproducts": P, "masses_sqr": [mass] * len(pairs),
-            "num_eps_terms": 1, "lambda": 0.0, "N": N, "seed": seed}
+            "num_eps_terms": 1, "lambda": lam if minkowski else 0.0, "N": N, "seed": seed}
 
 
 CONFIGS = {
@@ -219,6 +225,7 @@ LARGE = {
     "ladder5": lambda N=10**7, seed=0: ladder_4pt(5, N, seed),
     "ladder6": lambda N=10**7, seed=0: ladder_4pt(6, N, seed),
     "ladder7": lambda N=10**8, seed=0: ladder_4pt(7, N, seed),
+    "ladder4_mink": lambda N=10**7, seed=0: ladder_4pt(4, N, seed, D=6, minkowski=True),
 }
 
 

@@ -95,3 +95,26 @@ def test_large_graph_estimate_equals_oracle(engine, oracle):
     sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
     assert abs(s1[0, 0] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn)
     assert abs(s2[0, 0] - rs2) <= 1e-9 * abs(rs2)
+
+
+def test_large_deformed_graph_estimate_equals_oracle(engine, oracle):
+    """13-edge, 10-vertex Minkowski ladder (deformed complex path, row-table
+    placement): integrate(N, seed) draws the oracle's one-thread samples
+    (sample count and drops exact, real and imaginary means within 1e-6
+    sigma; second moments within 1e-7: they are dominated by a few large
+    samples, whose unpivoted complex factorisations differ from the pivoted
+    CPU ones in the last ~8 bits)."""
+    prob = W.LARGE["ladder4_mink"](N=50_000, seed=3)
+    p = engine.prepare(prob)
+    assert p.info.path == 2
+    op = oracle.problem(prob)
+    ref, dropped = op.estimate(prob["N"], prob["seed"], 1, 1)
+    m = engine.integrate(p, prob["N"], prob["seed"])
+    n, s1, s2 = m.arrays()
+    assert m.dropped == dropped
+    for q in (0, 1):
+        rn, rs1, rs2 = ref[0, q]
+        assert n[0, q] == rn
+        sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+        assert abs(s1[0, q] - rs1) / rn <= 1e-6 * sig + 1e-15 * abs(rs1 / rn), q
+        assert abs(s2[0, q] - rs2) <= 1e-7 * abs(rs2), q
