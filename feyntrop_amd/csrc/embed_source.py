@@ -1,5 +1,5 @@
 """Build step: embed the device headers into the runtime library as C strings:
-ftr_abi.hpp + ftr_device.hpp (the source hipRTC specialises per graph) and
+ftr_abi.hpp + ftr_math.hpp + ftr_device.hpp (the source hipRTC specialises per graph) and
 ftr_mlp.hpp (the learned proposal's network) and ftr_jtable.hpp (the
 J-table), each compiled once."""
 import os
@@ -7,9 +7,10 @@ import sys
 
 here = os.path.dirname(os.path.abspath(__file__))
 abi = open(os.path.join(here, "kernels", "ftr_abi.hpp")).read()
+math = open(os.path.join(here, "kernels", "ftr_math.hpp")).read()
 dev = open(os.path.join(here, "kernels", "ftr_device.hpp")).read()
-dev = dev.replace('#include "ftr_abi.hpp"', "")
-src = (abi + "\n" + dev).replace("#pragma once", "")
+dev = dev.replace('#include "ftr_abi.hpp"', "").replace('#include "ftr_math.hpp"', "")
+src = (abi + "\n" + math + "\n" + dev).replace("#pragma once", "")
 delim = "FTRSRC"
 assert (")" + delim) not in src
 mlp = open(os.path.join(here, "kernels", "ftr_mlp.hpp")).read().replace("#pragma once", "")

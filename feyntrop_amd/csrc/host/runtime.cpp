@@ -70,9 +70,11 @@ Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit)
   const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
   const size_t cap0 = std::min<size_t>(lds_limit, 64 * 1024);
   v.block = 256;
-  if (pr.path == 2 && rows_bytes(E) <= mem_limit) {
-    /* deformed path: the row layout (all E weights of a state in flight at
-     * once) beats the LDS scans (measured +5..9 % on 5- to 10-edge graphs) */
+  if (rows_bytes(E) <= mem_limit) {
+    /* the row layout (all E weights of a state in flight at once, the cut
+     * found by counting) beats the LDS scans: deformed path +5..9 % on 5- to
+     * 10-edge graphs, real path +2 % (K4 wheel) and +7 % (12-edge cubic
+     * graph, against w in LDS with 1024-thread blocks) */
     v.tmode = 3;
   } else if (full <= cap0) {
     v.tmode = 0;

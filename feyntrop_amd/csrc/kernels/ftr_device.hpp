@@ -39,6 +39,7 @@
 #include <stdint.h>
 #endif
 #include "ftr_abi.hpp"
+#include "ftr_math.hpp"
 
 
 namespace ftr {
@@ -77,38 +78,11 @@ __device__ inline void apply_poly(Xoshiro& g, const uint64_t* q) {
 }
 
 /* ---------------------------------------------------------- elementary -- */
-/* Natural log of a normal y > 0, without special cases (the samplers call it
- * on 1 - U, which lies in [2^-53, 1]).  The reduction is fdlibm's (e_log.c):
- * y = m 2^k with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f / (2 + f), and
- *   log(1 + f) = f - (f^2/2 - s (f^2/2 + R)),  R = 2 atanh(s)/s - 2 = z P(z),
- * z = s^2.  P has degree 6, fitted by a weighted Remez exchange on
- * z in [0, 0.02944] (error of R below 2.5e-18).  s comes from v_rcp_f64 and
- * two Newton steps.  Error below 0.82 ulp (3e5 arguments, against a 40-digit
- * log), in about 35 instructions; the library log takes about 80. */
-__device__ inline double log_pos(double y) {
-  double m = __builtin_amdgcn_frexp_mant(y);
-  int k = __builtin_amdgcn_frexp_exp(y);
-  if (m < 0x1.6a09e667f3bcdp-1) {
-    m += m;
-    k -= 1;
-  }
-  const double f = m - 1.0, d = 2.0 + f;
-  double r = __builtin_amdgcn_rcp(d);
-  r = fma(r, fma(-d, r, 1.0), r);
-  r = fma(r, fma(-d, r, 1.0), r);
-  const double s = f * r, z = s * s;
-  double P = 0x1.2f0563862e5fdp-3;
-  P = fma(z, P, 0x1.39a1bab73fcacp-3);
-  P = fma(z, P, 0x1.74663ee86e841p-3);
-  P = fma(z, P, 0x1.c71c52095d16fp-3);
-  P = fma(z, P, 0x1.24924941f124fp-2);
-  P = fma(z, P, 0x1.999999997fdb7p-2);
-  P = fma(z, P, 0x1.5555555555592p-1);
-  const double hfsq = 0.5 * f * f, dk = (double)k;
-  /* log 2 = 0x1.62e42p-1 (k times it is exact) + 0x1.fdf473de6af28p-22 */
-  const double t = fma(s, fma(z, P, hfsq), dk * 0x1.fdf473de6af28p-22);
-  return fma(dk, 0x1.62e42p-1, f - (hfsq - t));
-}
+using ftrm::atan2_fast;
+using ftrm::log_any;
+using ftrm::log_pos;
+using ftrm::rcp_nr;
+using ftrm::sincos_fast;
 
 /* ------------------------------------------------------------- complex -- */
 struct cplx {
@@ -131,21 +105,34 @@ __host__ __device__ inline cplx fms(cplx y, cplx a, cplx b) {
 }
 __host__ __device__ inline double fms(double y, double a, double b) { return fma(-a, b, y); }
 __host__ __device__ inline cplx crecip(cplx a) {
-  const double inv = 1.0 / (a.re * a.re + a.im * a.im);
+  const double inv = rcp_nr(a.re * a.re + a.im * a.im);
   return cplx{a.re * inv, -a.im * inv};
 }
 /* principal log: (log|z|, arg z) */
 __host__ __device__ inline cplx clog_(cplx z) {
   return cplx{log(hypot(z.re, z.im)), atan2(z.im, z.re)};
 }
+/* The same without branches from log_pos and atan2_fast: z is scaled by
+ * 2^-e, e the exponent of max(|re|, |im|), so that |z 2^-e|^2 lies in
+ * [1/4, 2), and log|z| = log(|z 2^-e|^2)/2 + e log 2.  z = 0 gives -inf,
+ * infinite or NaN parts give inf or NaN, as clog_ does. */
+__device__ inline cplx clog_fast(cplx z) {
+  const double mx = fmax(fabs(z.re), fabs(z.im));
+  const int e = __builtin_amdgcn_frexp_exp(mx);
+  const double zr = __builtin_ldexp(z.re, -e), zi = __builtin_ldexp(z.im, -e);
+  double lr = fma(0.5, log_pos(fma(zr, zr, zi * zi)), (double)e * 0x1.62e42fefa39efp-1);
+  if (mx == 0.0) lr = -__builtin_inf();
+  if (!__builtin_isfinite(mx)) lr = mx;
+  return cplx{lr, atan2_fast(z.im, z.re)};
+}
 /* libstdc++ pow(complex<double>, double): real fast path for positive reals,
  * else polar(exp(y Re log z), y Im log z)  (complex:1028-1039) */
 __host__ __device__ inline cplx cpow_real(cplx z, double y) {
   if (z.im == 0.0 && z.re > 0.0) return cplx{pow(z.re, y), 0.0};
-  const cplx t = clog_(z);
+  const cplx t = clog_fast(z);
   const double rho = exp(y * t.re);
   double s, c;
-  sincos(y * t.im, &s, &c);
+  sincos_fast(y * t.im, &s, &c);
   return cplx{rho * c, rho * s};
 }
 __host__ __device__ inline bool cfinite(cplx z) { return __builtin_isfinite(z.re) && __builtin_isfinite(z.im); }
@@ -288,7 +275,7 @@ __host__ __device__ inline double ldlt_inplace(double (&M)[V * (V + 1) / 2], dou
   for (int k = 0; k < V; k++) {
     const double d = M[FTR_TRI(k, k)];
     prod *= d;
-    const double inv = 1.0 / d;
+    const double inv = rcp_nr(d);
     dinv[k] = inv;
 #pragma unroll
     for (int i = k + 1; i < V; i++) {
@@ -397,7 +384,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
 #pragma unroll
   for (int e = 0; e < E; e++) {
     X[e] *= scale;
-    Xi[e] = 1.0 / X[e];
+    Xi[e] = rcp_nr(X[e]);
   }
   double xprod = 1.0;
 #pragma unroll
@@ -441,7 +428,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     if (!__builtin_isfinite(R)) return false;
     out[0] = cmk(R, 0.0);
     if (K > 1) {
-      const double ef = log(psi) - p.Lf * log(pxi);
+      const double ef = log_any(psi) - p.Lf * log_any(pxi);
 #pragma unroll
       for (int k = 1; k < K; k++) {
         R *= ef / (double)k;
@@ -560,7 +547,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     for (int e = 0; e < E; e++) {
       th[e] = (th[e] + p.m2[e]) * p.def_ref;
       double sn, cs;
-      sincos(th[e], &sn, &cs);
+      sincos_fast(th[e], &sn, &cs);
       cX[e] = cmk(X[e] * cs, X[e] * sn);
       cXi[e] = cmk(Xi[e] * cs, -Xi[e] * sn);
       xne += p.nu[e] * th[e];
@@ -598,7 +585,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     const cplx cPXi = cdot - ctr;
     cplx cR = p.IGtr * cpow_mD(p, cPsi);
     double sn, cs;
-    sincos(xne, &sn, &cs);
+    sincos_fast(xne, &sn, &cs);
     cR = cR * (jdet * cmk(cs, sn));
     cR = cR * cpow_mW(p, cPXi);
     /* Branch-free tail: the finiteness flag is returned, not branched on
@@ -607,7 +594,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     const bool ok = cfinite(cR);
     out[0] = cR;
     if (K > 1) {
-      const cplx lp = clog_(cPsi), lx = clog_(cPXi);
+      const cplx lp = clog_fast(cPsi), lx = clog_fast(cPXi);
       const cplx ef = lp - p.Lf * lx;
 #pragma unroll
       for (int k = 1; k < K; k++) {

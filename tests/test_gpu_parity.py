@@ -162,15 +162,17 @@ def test_estimate_equals_reference_one_thread(engine, golden_problems, name):
             assert abs(s2[k, q] - rs2) <= 1e-9 * abs(rs2) + 1e-300, (k, q)
 
 
-@pytest.mark.parametrize("name", ["triple_ladder", "cubic5"])
-def test_row_tables_equal_reference_one_thread(engine, golden_problems, name):
-    """Table mode 3 (per-state rows of cut weights in HBM) draws the same
-    samples as the other placements: the reference's one-thread moments;
-    sampler points equal the golden ones."""
-    print(f"[rows] {name}", flush=True)
+@pytest.mark.parametrize("name,tmode", [("triple_ladder", 3), ("cubic5", 3), ("cubic5", 1), ("wheel_k4", 0),
+                                        ("double_box", 0), ("double_box", 2)])
+def test_table_placements_equal_reference_one_thread(engine, golden_problems, name, tmode):
+    """Every J-table placement draws the reference's samples: rows of cut
+    weights in HBM (3), whole table in LDS (0), w in LDS (1), all from
+    global (2).  The reference's one-thread moments; sampler points equal
+    the golden ones."""
+    print(f"[placement] {name} table_mode {tmode}", flush=True)
     p = engine.prepare(W.all_problems()[name]())
-    engine.set_variant(p, 3, 256)
-    assert engine.variant(p) == (3, 256)
+    engine.set_variant(p, tmode, 256)
+    assert engine.variant(p) == (tmode, 256)
     g = golden_problems[name]["estimate"]
     N = g["N"]
     ref = _hex(g["moments"]).reshape(p.K, 2, 3)
