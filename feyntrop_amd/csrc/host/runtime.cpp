@@ -73,7 +73,7 @@ Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit)
   if (rows_bytes(E) <= mem_limit) {
     /* the row layout (all E weights of a state in flight at once, the cut
      * found by counting) beats the LDS scans: deformed path +5..9 % on 5- to
-     * 10-edge graphs, real path +2 % (K4 wheel) and +7 % (12-edge cubic
+     * 10-edge graphs, real path +2 % (K4 wheel) and +7 % (14-edge cubic
      * graph, against w in LDS with 1024-thread blocks) */
     v.tmode = 3;
   } else if (full <= cap0) {
@@ -99,7 +99,7 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
   const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
   for (int tm = 0; tm < 4; tm++)
     for (int blk : {256, 512, 1024}) {
-      const size_t t = tm == 0 ? full : (tm == 1 ? wonly : 0);
+      const size_t t = tm == 0 ? full : (tm == 1 ? wonly : (tm == 3 ? (size_t)E * blk * 8 : 0));
       if (t + (size_t)(blk / 64) * (2 + 4 * pr.K) * 8 + 16 > lds_limit) continue;
       if (tm == 0 && full > 64 * 1024 && blk == 256) continue;
       if (tm == 3 && rows_bytes(E) > mem_limit) continue;
@@ -116,6 +116,8 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
 size_t variant_lds(const Variant& v, int E) {
   size_t t = v.tmode == 0 ? ((size_t)25 << E) + 64 : (v.tmode == 1 ? ((size_t)8 << E) : 0);
   t = (t + 15) & ~(size_t)15;
+  /* table mode 3: each thread's E slots for the sampler's scatter of X */
+  if (v.tmode == 3) t += (size_t)E * v.block * 8;
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
 }
 

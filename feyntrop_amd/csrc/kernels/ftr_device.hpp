@@ -240,8 +240,12 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
     g = c;
     if (MM && !mm) lGv = lX;
     MM = mm;
+    if constexpr (TV::TM == 3) {
+      tv.xs[(size_t)e * blockDim.x] = XX;
+    } else {
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
+      for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
+    }
     if (L < oL) {
       lF += lX;
       oL = L;
@@ -251,6 +255,10 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
       lX += ir * log_pos(1. - U);
       XX = exp(lX);
     }
+  }
+  if constexpr (TV::TM == 3) {
+#pragma unroll
+    for (int k = 0; k < E; k++) X[k] = tv.xs[(size_t)k * blockDim.x];
   }
   lpsi_tr = lF;
   lG = lGv;
@@ -645,6 +653,7 @@ struct TableView<3> {
   const double* rows;
   const StateRec* rec;
   int E;
+  double* xs; /* this thread's LDS slots for X: xs[e * blockDim.x] */
   __device__ inline double w(uint32_t g) const { return pw[g]; }
   __device__ inline double jr(uint32_t g) const { return rec[g].jr; }
   __device__ inline double ir(uint32_t g) const { return rec[g].ir; }
@@ -656,6 +665,22 @@ template <int TMODE>
 __host__ __device__ constexpr size_t table_lds_bytes(int E) {
   return TMODE == 0 ? ((size_t)25 << E) + 64 : (TMODE == 1 ? ((size_t)8 << E) : 0);
 }
+/* With the tables in HBM (TMODE 3) the LDS holds each thread's E slots of X,
+ * where the sampler scatters the radius of the cut edge (one store per cut
+ * instead of a select over all E edges); layout [e][thread]. */
+template <int TMODE>
+__device__ inline size_t xs_lds_bytes(int E) {
+  return TMODE == 3 ? (size_t)E * blockDim.x * 8 : 0;
+}
+template <int TMODE>
+__device__ inline double* xs_slots(char* lds, int E) {
+  return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + threadIdx.x : nullptr;
+}
+/* start of the block reduction's LDS area */
+template <int TMODE>
+__device__ inline double* red_area(char* lds, int E) {
+  return reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E) + xs_lds_bytes<TMODE>(E));
+}
 
 /* Stage the table into LDS and build the view. */
 template <int TMODE>
@@ -664,6 +689,7 @@ __device__ inline TableView<TMODE> stage_table(const Launch& a, int E, char* lds
   const uint32_t NT = 1u << E;
   if constexpr (TMODE == 3) {
     tv.pw = a.t_w; tv.rows = a.t_rows; tv.rec = a.t_rec; tv.E = E;
+    tv.xs = xs_slots<3>(lds, E);
     return tv;
   } else if constexpr (TMODE == 2) {
     tv.pw = a.t_w; tv.pjr = a.t_jr; tv.pir = a.t_ir; tv.pmm = a.t_mmL;
@@ -724,7 +750,7 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   const ParamsFor<PT> ps{};
   const PT& p = ps.get(a);
   TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
-  double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
+  double* red = red_area<TMODE>(lds, E);
 
   Xoshiro rng = lane_stream(a, bid);
   const uint64_t lane = (uint64_t)bid * blockDim.x + threadIdx.x;
@@ -826,7 +852,8 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
  * absent edges leave the sum unchanged). */
 template <class G, bool KEEP_LY>
 __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&X)[G::E], double& lpsi,
-                                   double& lG, double& wgt, uint64_t& path, double* ly_out, uint64_t ly_stride) {
+                                   double& lG, double& wgt, uint64_t& path, double* ly_out, uint64_t ly_stride,
+                                   double* xs) {
   constexpr int E = G::E;
   uint32_t g = (1u << E) - 1u;
   LQuad st = L.lst[g];
@@ -843,18 +870,15 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     double rv[E];
 #pragma unroll
     for (int q = 0; q < E; q++) rv[q] = row[q];
+    /* first q with T_q > R = the number of nondecreasing sums <= R */
     double T = 0.0;
-    int e = 0, last = 0;
-    bool found = false;
+    int cnt = 0;
 #pragma unroll
     for (int q = 0; q < E; q++) {
       T += rv[q];
-      if ((g >> q) & 1u) last = q;
-      const bool hit = !found && T > R;
-      if (hit) e = q;
-      found = found || hit;
+      cnt += (T <= R) ? 1 : 0;
     }
-    if (!found) e = last;
+    const int e = (cnt < E) ? cnt : 31 - __builtin_clz(g);
     const uint32_t c = g ^ (1u << e);
     ratio *= st.A * L.ltw[(size_t)g * E + e];
     st = L.lst[c];
@@ -862,8 +886,12 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     const int Lc = (int)(st.mmL >> 1);
     if (MM && !mm) lGv = lX;
     MM = mm;
+    if (xs) {
+      xs[(size_t)e * blockDim.x] = XX;
+    } else {
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
+      for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
+    }
     if (Lc < oL) {
       lF += lX;
       oL = Lc;
@@ -878,6 +906,10 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
       if (KEEP_LY) ly_out[(size_t)j * ly_stride] = lY;
     }
     g = c;
+  }
+  if (xs) {
+#pragma unroll
+    for (int k = 0; k < E; k++) X[k] = xs[(size_t)k * blockDim.x];
   }
   lpsi = lF;
   lG = lGv;
@@ -896,8 +928,10 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   constexpr int NQ = 2 + 4 * K;
   const Launch& a = L.a;
   const DevParams& p = *a.prm;
-  /* the learned tables are read from L2/HBM: the LDS holds the reduction */
-  double* red = reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E));
+  /* the learned tables are read from L2/HBM: the LDS holds the X slots
+   * (TMODE 3) and the reduction */
+  double* red = red_area<TMODE>(lds, E);
+  double* xs = xs_slots<TMODE>(lds, E);
   Xoshiro rng = lane_stream(a, blockIdx.x);
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
@@ -908,7 +942,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, GRAD>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N);
+    learn_point<G, GRAD>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
     if (GRAD) L.gpath[i] = path;
     cplx out[K];
     if (!integrand<G, K, MODE>(p, X, log_scale(p, lF, lGv), out)) {
@@ -1125,7 +1159,7 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, false>(L, rng, X, lF, lGv, wgt, path, nullptr, 0);
+    learn_point<G, false>(L, rng, X, lF, lGv, wgt, path, nullptr, 0, xs_slots<TMODE>(lds, E));
 #pragma unroll
     for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
     wout[i] = wgt;
