@@ -38,8 +38,8 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # Summary and commands under profiles/.
 _FETCH_SCALE = 262144 / 131087.0625
 PMC_TRAFFIC_BYTES = {
-    ("double_box", "generic"): (404.4 * _FETCH_SCALE + 24.0) * 1024,
-    ("double_box", "specialised"): (372.5 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "generic"): (427.4 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "specialised"): (375.0 * _FETCH_SCALE + 24.0) * 1024,
 }
 
 

@@ -241,7 +241,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
     if (MM && !mm) lGv = lX;
     MM = mm;
     if constexpr (TV::TM == 3) {
-      tv.xs[(size_t)e * blockDim.x] = XX;
+      tv.xs[e] = XX;
     } else {
 #pragma unroll
       for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
@@ -258,7 +258,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
   }
   if constexpr (TV::TM == 3) {
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = tv.xs[(size_t)k * blockDim.x];
+    for (int k = 0; k < E; k++) X[k] = tv.xs[k];
   }
   lpsi_tr = lF;
   lG = lGv;
@@ -653,7 +653,7 @@ struct TableView<3> {
   const double* rows;
   const StateRec* rec;
   int E;
-  double* xs; /* this thread's LDS slots for X: xs[e * blockDim.x] */
+  double* xs; /* this thread's E LDS slots for X */
   __device__ inline double w(uint32_t g) const { return pw[g]; }
   __device__ inline double jr(uint32_t g) const { return rec[g].jr; }
   __device__ inline double ir(uint32_t g) const { return rec[g].ir; }
@@ -667,14 +667,16 @@ __host__ __device__ constexpr size_t table_lds_bytes(int E) {
 }
 /* With the tables in HBM (TMODE 3) the LDS holds each thread's E slots of X,
  * where the sampler scatters the radius of the cut edge (one store per cut
- * instead of a select over all E edges); layout [e][thread]. */
+ * instead of a select over all E edges); layout [thread][e], so that the
+ * reads are one base register plus constant offsets. */
 template <int TMODE>
 __device__ inline size_t xs_lds_bytes(int E) {
   return TMODE == 3 ? (size_t)E * blockDim.x * 8 : 0;
 }
 template <int TMODE>
 __device__ inline double* xs_slots(char* lds, int E) {
-  return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + threadIdx.x : nullptr;
+  return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + (size_t)threadIdx.x * E
+                     : nullptr;
 }
 /* start of the block reduction's LDS area */
 template <int TMODE>
@@ -755,12 +757,13 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   Xoshiro rng = lane_stream(a, bid);
   const uint64_t lane = (uint64_t)bid * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
-  const uint64_t i1 = (i0 + a.S < a.N) ? i0 + a.S : a.N;
+  /* this lane's sample count (one loop register instead of two bounds) */
+  const uint64_t n = (i0 < a.N) ? ((a.N - i0 < a.S) ? a.N - i0 : a.S) : 0;
 
   double acc[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) acc[q] = 0.0;
-  for (uint64_t i = i0; i < i1; i++) {
+  for (uint64_t j = n; j > 0; j--) {
     double X[E], lpsi, lG;
     sample_point<G>(rng, tv, X, lpsi, lG);
     cplx out[K];
@@ -887,7 +890,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     if (MM && !mm) lGv = lX;
     MM = mm;
     if (xs) {
-      xs[(size_t)e * blockDim.x] = XX;
+      xs[e] = XX;
     } else {
 #pragma unroll
       for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
@@ -909,7 +912,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
   }
   if (xs) {
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = xs[(size_t)k * blockDim.x];
+    for (int k = 0; k < E; k++) X[k] = xs[k];
   }
   lpsi = lF;
   lG = lGv;
