@@ -102,6 +102,10 @@ void ensure_scratch(ftr_ctx* c, size_t bytes);
 void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
 Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit = (size_t)128 << 30);
 size_t variant_lds(const Variant& v, int E);
+/* the variant keeps the deformation's second derivatives in LDS (FTR_DD_LDS_MAX_E) */
+inline bool dd_in_lds(const Variant& v, int E) {
+  return v.mode == 2 && v.tmode == 3 && v.cparams && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
+}
 int fail(ftr_ctx* c, int code, const std::string& msg);
 /* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */
 void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& res);

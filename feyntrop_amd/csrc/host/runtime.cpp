@@ -118,6 +118,7 @@ size_t variant_lds(const Variant& v, int E) {
   t = (t + 15) & ~(size_t)15;
   /* table mode 3: each thread's E slots for the sampler's scatter of X */
   if (v.tmode == 3) t += (size_t)E * v.block * 8;
+  if (dd_in_lds(v, E)) t += (size_t)(E * (E + 1) / 2) * v.block * 8;
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
 }
 
@@ -164,6 +165,7 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
     s << "};\n";
   }
   s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
+  s << "  static constexpr bool DDL = " << (dd_in_lds(v, g.E) ? "true" : "false") << ";\n";
   s << (v.cparams ? "  using PRM = FP;\n" : "  using PRM = ftr::DevParams;\n");
   arr(s, "hi", g.hi);
   arr(s, "lo", g.lo);
@@ -1121,6 +1123,7 @@ int ftr_set_specialized(ftr_ctx* c, ftr_problem* p, int on) {
     v.cparams = on ? 1 : 0;
     p->m = load_module(c, full_source(p->prep, v));
     p->var = v;
+    p->lds = variant_lds(v, p->prep.g.E);
     if (p->lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
   } catch (std::exception& e) {

@@ -15,6 +15,12 @@ typedef __hip_internal::uint64_t uint64_t;
 
 #define FTR_MAX_E 30
 #define FTR_MAX_V 24
+/* Kernels specialised to a kinematic point on the deformed path with the
+ * row-table placement, 256-thread blocks and E <= FTR_DD_LDS_MAX_E keep the
+ * second derivatives of the deformation (E(E+1)/2 doubles per thread) in LDS
+ * instead of registers (measured: 12.1 -> 11.6 ms per 1e8 double-box
+ * samples; slower at E = 10 and for the generic kernel). */
+#define FTR_DD_LDS_MAX_E 8
 
 namespace ftr {
 

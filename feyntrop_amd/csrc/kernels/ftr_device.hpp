@@ -366,9 +366,9 @@ __host__ __device__ inline double log_scale(const PT& p, double lpsi, double lG)
 /* PT: the problem's parameters, ftr::DevParams read from memory or a type
  * with the same members as compile-time constants (kernels specialised to
  * one kinematic point). */
-template <class G, int K, int MODE, class PT = DevParams>
+template <class G, int K, int MODE, class PT = DevParams, bool DDL = false>
 __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double lscale,
-                                          cplx (&out)[K]) {
+                                          cplx (&out)[K], double* dd_slots = nullptr) {
   constexpr int E = G::E, V = G::V, NT = V * (V + 1) / 2;
   /* vertex weights and v* (first minimum) -- feynman_integral.hpp:166-194 */
   double XV[V];
@@ -508,7 +508,9 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     }
     /* first and second derivatives of pphi (laplacian.hpp:102-215) */
     double th[E];
-    double dd[E * (E + 1) / 2];
+    /* in registers, or (DDL) in this thread's LDS slots */
+    double dd_regs[DDL ? 1 : E * (E + 1) / 2];
+    double* const dd = DDL ? dd_slots : dd_regs;
 #pragma unroll
     for (int a = 0; a < E; a++) {
       const int ka = G::pos[G::hi[a]], la = G::pos[G::lo[a]];
@@ -686,10 +688,23 @@ __device__ inline double* xs_slots(char* lds, int E) {
   return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + (size_t)threadIdx.x * E
                      : nullptr;
 }
+/* the deformation's second derivatives in LDS (G::DDL): E(E+1)/2 slots per
+ * thread after the X slots, layout [thread][entry] */
+template <class G>
+__device__ inline size_t dd_lds_bytes() {
+  return G::DDL ? (size_t)(G::E * (G::E + 1) / 2) * blockDim.x * 8 : 0;
+}
+template <class G, int TMODE>
+__device__ inline double* dd_slots(char* lds) {
+  return G::DDL ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(G::E) + xs_lds_bytes<TMODE>(G::E)) +
+                      (size_t)threadIdx.x * (G::E * (G::E + 1) / 2)
+                : nullptr;
+}
 /* start of the block reduction's LDS area */
-template <int TMODE>
-__device__ inline double* red_area(char* lds, int E) {
-  return reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E) + xs_lds_bytes<TMODE>(E));
+template <class G, int TMODE>
+__device__ inline double* red_area(char* lds) {
+  return reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(G::E) + xs_lds_bytes<TMODE>(G::E) +
+                                   dd_lds_bytes<G>());
 }
 
 /* Stage the table into LDS and build the view. */
@@ -760,7 +775,8 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   const ParamsFor<PT> ps{};
   const PT& p = ps.get(a);
   TableView<TMODE> tv = stage_table<TMODE>(a, E, lds);
-  double* red = red_area<TMODE>(lds, E);
+  double* red = red_area<G, TMODE>(lds);
+  double* const dds = dd_slots<G, TMODE>(lds);
 
   Xoshiro rng = lane_stream(a, bid);
   const uint64_t lane = (uint64_t)bid * blockDim.x + threadIdx.x;
@@ -775,7 +791,7 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
     double X[E], lpsi, lG;
     sample_point<G>(rng, tv, X, lpsi, lG);
     cplx out[K];
-    if (!integrand<G, K, MODE, PT>(p, X, log_scale(p, lpsi, lG), out)) {
+    if (!integrand<G, K, MODE, PT, G::DDL>(p, X, log_scale(p, lpsi, lG), out, dds)) {
       acc[1] += 1.0;
       continue;
     }
@@ -941,8 +957,9 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   const DevParams& p = *a.prm;
   /* the learned tables are read from L2/HBM: the LDS holds the X slots
    * (TMODE 3) and the reduction */
-  double* red = red_area<TMODE>(lds, E);
+  double* red = red_area<G, TMODE>(lds);
   double* xs = xs_slots<TMODE>(lds, E);
+  double* const dds = dd_slots<G, TMODE>(lds);
   Xoshiro rng = lane_stream(a, blockIdx.x);
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t i0 = lane * a.S;
@@ -956,7 +973,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
     learn_point<G, GRAD>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
     if (GRAD) L.gpath[i] = path;
     cplx out[K];
-    if (!integrand<G, K, MODE>(p, X, log_scale(p, lF, lGv), out)) {
+    if (!integrand<G, K, MODE, DevParams, G::DDL>(p, X, log_scale(p, lF, lGv), out, dds)) {
       acc[1] += 1.0;
       if (GRAD) L.gbuf[i] = 0.0;
       continue;
