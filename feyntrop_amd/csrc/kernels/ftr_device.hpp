@@ -887,6 +887,9 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
   bool MM = st.mmL & 1;
   int oL = (int)(st.mmL >> 1);
   double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0, ratio = 1.0, lrad = 0.0;
+  /* the cut's ratio factor is a load that depends on the cut: it is
+   * multiplied in one step later (the same products in the same order) */
+  double fpend = 1.0;
   path = 0;
 #pragma unroll
   for (int e = 0; e < E; e++) X[e] = 0.0;
@@ -907,7 +910,8 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     }
     const int e = (cnt < E) ? cnt : 31 - __builtin_clz(g);
     const uint32_t c = g ^ (1u << e);
-    ratio *= st.A * L.ltw[(size_t)g * E + e];
+    ratio *= fpend;
+    fpend = st.A * L.ltw[(size_t)g * E + e];
     st = L.lst[c];
     const bool mm = st.mmL & 1;
     const int Lc = (int)(st.mmL >> 1);
@@ -940,7 +944,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
   }
   lpsi = lF;
   lG = lGv;
-  wgt = ratio * exp(lrad);
+  wgt = (ratio * fpend) * exp(lrad);
 }
 
 /* Integration with the learned proposal.  GRAD (training) also records
