@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,8 @@ struct ftr_ctx {
   std::string err;
   std::map<std::string, Module> modules;
   std::map<std::string, hipModule_t> raw_modules; /* every loaded code object */
+  std::map<std::string, Module> spec_modules;     /* the same modules, keyed by the graph spec alone */
+  std::map<std::string, std::pair<int, int>> tune_memo; /* tuning file -> (table mode, block), -1: none */
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   /* growable device scratch: block states, lane polys, partials */
   void* d_scratch = nullptr;
@@ -63,6 +66,9 @@ struct ftr_problem {
   ftr::DevParams hprm;
   void* d_slab = nullptr; /* one allocation: parameters and sampler tables */
   void* d_rows_slab = nullptr; /* table mode 3: rows and state records */
+  /* problems prepared together share one allocation (d_slab and
+   * d_rows_slab then stay null; the last holder frees it) */
+  std::shared_ptr<void> d_shared;
   double* d_rows = nullptr;
   ftr::StateRec* d_rec = nullptr;
   ftr::DevParams* d_prm = nullptr;
@@ -109,8 +115,12 @@ inline bool dd_in_lds(const Variant& v, int E) {
 int fail(ftr_ctx* c, int code, const std::string& msg);
 /* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */
 void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& res);
-/* build the row tables of table mode 3 on the device (jtable_gpu.cpp) */
+/* build the row tables of table mode 3 on the device (jtable_gpu.cpp):
+ * device_rows allocates and waits; launch_rows fills rows_region_bytes(E)
+ * at d on the context's stream without waiting */
 void device_rows(ftr_ctx* c, ftr_problem* p);
+size_t rows_region_bytes(int E);
+void launch_rows(ftr_ctx* c, ftr_problem* p, char* d);
 /* bytes of the table-mode-3 tables of a graph with E edges */
 inline size_t rows_bytes(int E) { return ((size_t)1 << E) * ((size_t)8 * E + sizeof(ftr::StateRec)); }
 /* the table pointers of a problem into a launch */

@@ -139,23 +139,31 @@ void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::Jt
   pr.jtable_on_device = true;
 }
 
-void device_rows(ftr_ctx* c, ftr_problem* p) {
-  if (p->d_rows_slab) return;
+size_t rows_region_bytes(int E) {
+  const size_t NT = (size_t)1 << E;
+  return ((NT * E * 8 + 255) & ~(size_t)255) + NT * sizeof(ftr::StateRec);
+}
+
+void launch_rows(ftr_ctx* c, ftr_problem* p, char* d) {
   const int E = p->prep.g.E;
   const uint64_t NT = (uint64_t)1 << E;
-  HIPCHK(hipSetDevice(c->device));
   hipModule_t mod = load_module_raw(c, ftr_embedded_jtable_source);
   hipFunction_t f = nullptr;
   HIPCHK(hipModuleGetFunction(&f, mod, "ftrj_rows"));
-  const size_t rb = ((size_t)NT * E * 8 + 255) & ~(size_t)255;
-  char* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, rb + NT * sizeof(ftr::StateRec)));
-  p->d_rows_slab = d;
   p->d_rows = (double*)d;
-  p->d_rec = (ftr::StateRec*)(d + rb);
+  p->d_rec = (ftr::StateRec*)(d + ((NT * E * 8 + 255) & ~(size_t)255));
   uint64_t nt = NT;
   int e = E;
   void* args[] = {&nt, &e, &p->d_w, &p->d_jr, &p->d_ir, &p->d_mml, &p->d_rows, &p->d_rec};
   HIPCHK(hipModuleLaunchKernel(f, (unsigned)((NT + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+}
+
+void device_rows(ftr_ctx* c, ftr_problem* p) {
+  if (p->d_rows) return;
+  HIPCHK(hipSetDevice(c->device));
+  char* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, rows_region_bytes(p->prep.g.E)));
+  p->d_rows_slab = d;
+  launch_rows(c, p, d);
   HIPCHK(hipStreamSynchronize(c->stream));
 }

@@ -304,6 +304,9 @@ void host_jtable(Prep& pr, const JtSetup& js, JtResult& res) {
           for (int j = 0; j < E; j++)
             if (!((s >> j) & 1)) pxi += pr.m2[j] * Xs[(size_t)t * E + j];
           sum += std::fabs(pxi);
+          /* the sum only grows: once over the threshold the flag is settled
+           * (the reference sums all the test points; same flag) */
+          if (!(sum / pmsqr_ref < 1e-12)) break;
         }
         mm = sum / pmsqr_ref < 1e-12;
       } else if (n == E) {

@@ -256,12 +256,21 @@ static std::string tune_path(const ftrh::Prep& pr) {
   return cache_dir() + name;
 }
 
-static bool load_tuned(const ftrh::Prep& pr, size_t lds_limit, Variant* out) {
-  std::ifstream f(tune_path(pr));
-  int tm = -1, blk = 0;
-  if (!(f >> tm >> blk)) return false;
+/* The autotuner's recorded variant of a graph, through the context's memo of
+ * tuning files (one file read per graph and process; the autotuner drops the
+ * entry it rewrites). */
+static bool load_tuned_memo(ftr_ctx* c, const ftrh::Prep& pr, size_t lds_limit, Variant* out) {
+  const std::string path = tune_path(pr);
+  auto it = c->tune_memo.find(path);
+  if (it == c->tune_memo.end()) {
+    std::ifstream f(path);
+    int tm = -1, blk = 0;
+    if (!(f >> tm >> blk)) tm = -1;
+    it = c->tune_memo.emplace(path, std::make_pair(tm, blk)).first;
+  }
+  if (it->second.first < 0) return false;
   for (const Variant& v : candidate_variants(pr, lds_limit))
-    if (v.tmode == tm && v.block == blk) {
+    if (v.tmode == it->second.first && v.block == it->second.second) {
       *out = v;
       return true;
     }
@@ -348,6 +357,17 @@ static Module load_module(ftr_ctx* c, const std::string& src) {
   HIPCHK(hipModuleGetFunction(&m.f_smp, m.mod, "ftrk_smp"));
   HIPCHK(hipModuleGetFunction(&m.f_bat, m.mod, "ftrk_bat"));
   c->modules[src] = m;
+  return m;
+}
+
+/* The integration module of a problem's variant, looked up by the graph spec
+ * (a few hundred bytes) instead of the full kernel source. */
+static Module module_for(ftr_ctx* c, const ftrh::Prep& pr, const Variant& v) {
+  std::string key = spec_source(pr, v);
+  auto it = c->spec_modules.find(key);
+  if (it != c->spec_modules.end()) return it->second;
+  const Module m = load_module(c, full_source(pr, v));
+  c->spec_modules.emplace(std::move(key), m);
   return m;
 }
 
@@ -531,40 +551,104 @@ static void prepare_host(ftr_ctx* c, ftr_problem* p, const ftr_input* in) {
 /* Device phase: kernel module, and the parameters and sampler tables in one
  * allocation (DevParams | w | J | 1/r | mm|L<<1), uploaded with one copy
  * from a host staging buffer. */
-static void prepare_device(ftr_ctx* c, ftr_problem* p) {
-  HIPCHK(hipSetDevice(c->device));
-  if (!load_tuned(p->prep, c->prop.sharedMemPerBlock, &p->var))
+/* Variant, kernel module and parameters of a prepared problem. */
+static void choose_and_load(ftr_ctx* c, ftr_problem* p) {
+  if (!load_tuned_memo(c, p->prep, c->prop.sharedMemPerBlock, &p->var))
     p->var = choose_variant(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c));
   p->lds = variant_lds(p->var, p->prep.g.E);
-  p->m = load_module(c, full_source(p->prep, p->var));
+  p->m = module_for(c, p->prep, p->var);
   if (p->lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
   fill_params(p);
-  const size_t NT = (size_t)1 << p->prep.g.E;
-  const size_t o_w = (sizeof(ftr::DevParams) + 255) & ~(size_t)255;
-  const size_t o_jr = o_w + NT * 8, o_ir = o_jr + NT * 8, o_mml = o_ir + NT * 8, total = o_mml + NT;
-  std::vector<char> h(total);
-  memcpy(h.data(), &p->hprm, sizeof(ftr::DevParams));
-  double* w = (double*)(h.data() + o_w);
-  double* ir = (double*)(h.data() + o_ir);
-  uint8_t* mml = (uint8_t*)(h.data() + o_mml);
-  memcpy(h.data() + o_jr, p->prep.Jr.data(), NT * 8);
-  for (size_t i = 0; i < NT; i++) {
+}
+
+/* Layout of a problem's slab: DevParams | w | J | 1/r | mm|L<<1. */
+struct SlabLayout {
+  size_t NT, o_w, o_jr, o_ir, o_mml, total;
+  explicit SlabLayout(int E) {
+    NT = (size_t)1 << E;
+    o_w = (sizeof(ftr::DevParams) + 255) & ~(size_t)255;
+    o_jr = o_w + NT * 8;
+    o_ir = o_jr + NT * 8;
+    o_mml = o_ir + NT * 8;
+    total = o_mml + NT;
+  }
+};
+
+/* host image of the slab */
+static void stage_slab(const ftr_problem* p, const SlabLayout& L, char* h) {
+  memcpy(h, &p->hprm, sizeof(ftr::DevParams));
+  double* w = (double*)(h + L.o_w);
+  double* ir = (double*)(h + L.o_ir);
+  uint8_t* mml = (uint8_t*)(h + L.o_mml);
+  memcpy(h + L.o_jr, p->prep.Jr.data(), L.NT * 8);
+  for (size_t i = 0; i < L.NT; i++) {
     w[i] = p->prep.Jr[i] / p->prep.r[i];
     ir[i] = 1. / p->prep.r[i];
     mml[i] = (uint8_t)((p->prep.mm[i] ? 1 : 0) | (p->prep.Lt[i] << 1));
   }
-  char* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, total));
-  p->d_slab = d;
+}
+
+/* device pointers of a slab at d */
+static void bind_slab(ftr_problem* p, const SlabLayout& L, char* d) {
   p->d_prm = (ftr::DevParams*)d;
-  p->d_w = (double*)(d + o_w);
-  p->d_jr = (double*)(d + o_jr);
-  p->d_ir = (double*)(d + o_ir);
-  p->d_mml = (uint8_t*)(d + o_mml);
-  HIPCHK(hipMemcpyHtoDAsync(d, h.data(), total, c->stream));
+  p->d_w = (double*)(d + L.o_w);
+  p->d_jr = (double*)(d + L.o_jr);
+  p->d_ir = (double*)(d + L.o_ir);
+  p->d_mml = (uint8_t*)(d + L.o_mml);
+}
+
+static void prepare_device(ftr_ctx* c, ftr_problem* p) {
+  HIPCHK(hipSetDevice(c->device));
+  choose_and_load(c, p);
+  const SlabLayout L(p->prep.g.E);
+  std::vector<char> h(L.total);
+  stage_slab(p, L, h.data());
+  char* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, L.total));
+  p->d_slab = d;
+  bind_slab(p, L, d);
+  HIPCHK(hipMemcpyHtoDAsync(d, h.data(), L.total, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (p->var.tmode == 3) device_rows(c, p);
+}
+
+/* Device phase of many problems at once: one allocation shared by all
+ * slabs and row tables, one copy of the slabs, the row kernels back to
+ * back and one wait. */
+static void prepare_device_batch(ftr_ctx* c, const std::vector<ftr_problem*>& ps) {
+  if (ps.empty()) return;
+  HIPCHK(hipSetDevice(c->device));
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  std::vector<size_t> o_slab(ps.size()), o_rows(ps.size(), 0);
+  size_t slabs = 0;
+  for (size_t i = 0; i < ps.size(); i++) {
+    choose_and_load(c, ps[i]);
+    o_slab[i] = slabs;
+    slabs += al(SlabLayout(ps[i]->prep.g.E).total);
+  }
+  size_t total = slabs;
+  for (size_t i = 0; i < ps.size(); i++)
+    if (ps[i]->var.tmode == 3) {
+      o_rows[i] = total;
+      total += al(rows_region_bytes(ps[i]->prep.g.E));
+    }
+  std::vector<char> h(slabs);
+  for (size_t i = 0; i < ps.size(); i++) stage_slab(ps[i], SlabLayout(ps[i]->prep.g.E), h.data() + o_slab[i]);
+  char* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, total));
+  const int dev = c->device;
+  std::shared_ptr<void> own(d, [dev](void* q) {
+    (void)hipSetDevice(dev);
+    (void)hipFree(q);
+  });
+  HIPCHK(hipMemcpyHtoDAsync(d, h.data(), slabs, c->stream));
+  for (size_t i = 0; i < ps.size(); i++) {
+    ps[i]->d_shared = own;
+    bind_slab(ps[i], SlabLayout(ps[i]->prep.g.E), d + o_slab[i]);
+    if (ps[i]->var.tmode == 3) launch_rows(c, ps[i], d + o_rows[i]);
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 int ftr_prepare(ftr_ctx* c, const ftr_input* in, ftr_problem** out) {
@@ -627,23 +711,34 @@ int ftr_prepare_batch(ftr_ctx* c, int n, const ftr_input* in, ftr_problem** out,
       c->batch_errors[i] = e.what();
     }
   }
+  std::vector<ftr_problem*> ok;
   for (int i = 0; i < n; i++) {
     if (status[i] != FTR_OK) {
       ftr_problem_free(ps[i]);
-      continue;
-    }
-    try {
-      const auto t0 = std::chrono::steady_clock::now();
-      prepare_device(c, ps[i]);
-      ps[i]->prep.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      out[i] = ps[i];
-    } catch (std::exception& e) {
-      const Error* fe = dynamic_cast<const Error*>(&e);
-      status[i] = fe ? fe->code : FTR_ERR_DEVICE;
-      c->batch_errors[i] = e.what();
-      ftr_problem_free(ps[i]);
+      ps[i] = nullptr;
+    } else {
+      ok.push_back(ps[i]);
     }
   }
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    prepare_device_batch(c, ok);
+    const double share = ok.empty() ? 0.0
+                                    : std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() /
+                                          (double)ok.size();
+    for (ftr_problem* p : ok) p->prep.seconds += share;
+  } catch (std::exception& e) {
+    /* the device phase failed as a whole: every prepared problem fails */
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    for (int i = 0; i < n; i++)
+      if (ps[i]) {
+        status[i] = fe ? fe->code : FTR_ERR_DEVICE;
+        c->batch_errors[i] = e.what();
+        ftr_problem_free(ps[i]);
+        ps[i] = nullptr;
+      }
+  }
+  for (int i = 0; i < n; i++) out[i] = ps[i];
   return FTR_OK;
 }
 
@@ -1034,7 +1129,7 @@ int ftr_set_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block) {
       if (v.tmode == table_mode && v.block == block) {
         v.cparams = p->var.cparams;
         if (v.tmode == 3) device_rows(c, p);
-        p->m = load_module(c, full_source(p->prep, v));
+        p->m = module_for(c, p->prep, v);
         p->var = v;
         p->lds = variant_lds(v, p->prep.g.E);
         if (p->lds > 64 * 1024)
@@ -1085,6 +1180,7 @@ int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, 
     mkdir(cache_dir().c_str(), 0755);
     std::ofstream f(tune_path(p->prep));
     if (f) f << bestv.tmode << " " << bestv.block << "\n";
+    c->tune_memo.erase(tune_path(p->prep));
   }
   /* ... then the kernel specialised to this kinematic point */
   if (ftr_set_specialized(c, p, 1) == FTR_OK) {
@@ -1121,7 +1217,7 @@ int ftr_set_specialized(ftr_ctx* c, ftr_problem* p, int on) {
     HIPCHK(hipSetDevice(c->device));
     Variant v = p->var;
     v.cparams = on ? 1 : 0;
-    p->m = load_module(c, full_source(p->prep, v));
+    p->m = module_for(c, p->prep, v);
     p->var = v;
     p->lds = variant_lds(v, p->prep.g.E);
     if (p->lds > 64 * 1024)

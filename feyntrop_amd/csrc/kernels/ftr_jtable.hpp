@@ -197,6 +197,9 @@ __device__ inline void jt_level_body(const JtParams* __restrict__ prm, int n, ui
       for (int j = 0; j < E; j++)
         if (!((s >> j) & 1)) pxi += p.m2[j] * Xs[(size_t)tt * E + j];
       sum += fabs(pxi);
+      /* the sum only grows: once over the threshold the flag is settled
+       * (the reference sums all the test points; same flag) */
+      if (!(sum / p.pmsqr < 1e-12)) break;
     }
     mm = sum / p.pmsqr < 1e-12;
   } else if (n == E) {

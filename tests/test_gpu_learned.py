@@ -141,20 +141,28 @@ def _rel2(p, m):
 
 @pytest.mark.parametrize("name", ["triple_ladder", "double_box", "2L_3pt_tutorial", "cubic5"])
 def test_training_unbiased_and_lowers_variance(engine, name):
+    """Training keeps the estimate unbiased (within 3 combined sigma of the
+    plain sampler's) and lowers the variance.  The training accumulates with
+    float atomics, so the network differs from run to run at rounding level,
+    and a single relative-variance estimate of 2e7 samples is heavy-tailed on
+    the double box (single-seed ratios from 0.3 to 3.4 were seen): the gain
+    is the ratio of the medians over three seeds of each sampler (measured
+    3.6-5.3 over eight trainings on the double box)."""
     prob, p = _setup(engine, name)
     lr = feyntrop_amd.Learner(engine, p, init_seed=1)
     st = lr.train(steps=150, samples=1 << 18, seed=100, lr=3e-3)
     assert np.all(np.isfinite(st[:, 0]))
     N = 20_000_000
-    mp = engine.integrate(p, N, 21)
-    ml = lr.integrate(N, 22)
-    rp, rl = p.finalize(mp), p.finalize(ml)
+    mps = [engine.integrate(p, N, sd) for sd in (21, 25, 26)]
+    mls = [lr.integrate(N, sd) for sd in (22, 23, 24)]
+    rp, rl = p.finalize(mps[0]), p.finalize(mls[0])
     for k in range(p.K):
         for q in range(2 if p.info.path == 2 else 1):
             a, da = rl[k, q]
             b, db = rp[k, q]
             assert abs(a - b) <= 3.0 * np.hypot(da, db) + 1e-300, (k, q, a, da, b, db)
-    gain = _rel2(p, mp) / _rel2(p, ml)
+    gain = np.median([_rel2(p, m) for m in mps]) / np.median([_rel2(p, m) for m in mls])
+    print(f"[train] {name} gain {gain:.2f}", flush=True)
     assert gain > 1.5, gain
 
 
