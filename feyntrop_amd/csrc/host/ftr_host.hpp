@@ -72,6 +72,7 @@ void preprocess(Prep& pr);
 struct Poly { uint64_t w[4]; };
 Poly xo_jump_poly(uint64_t lo, uint64_t hi);   /* x^d mod charpoly */
 Poly xo_poly_square(const Poly& a);
+Poly xo_poly_mul(const Poly& a, const Poly& b); /* product mod charpoly: the composed jump */
 void xo_apply(const Poly& q, uint64_t* st);
 void xo_seed(uint64_t seed, uint64_t* st);
 void xo_charpoly(uint64_t* w5);

@@ -7,6 +7,7 @@
 
 #include <map>
 #include <memory>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,10 @@ struct ftr_ctx {
   std::map<std::string, hipModule_t> raw_modules; /* every loaded code object */
   std::map<std::string, Module> spec_modules;     /* the same modules, keyed by the graph spec alone */
   std::map<std::string, std::pair<int, int>> tune_memo; /* tuning file -> (table mode, block), -1: none */
+  /* jump polynomials per (S, 2E-1, log2 block, log2 blocks) and the kernel
+   * that chains the block start states */
+  std::map<std::tuple<uint64_t, uint64_t, int, int>, std::vector<uint64_t>> jump_cache;
+  hipFunction_t f_block_states = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   /* growable device scratch: block states, lane polys, partials */
   void* d_scratch = nullptr;

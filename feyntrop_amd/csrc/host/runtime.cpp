@@ -371,6 +371,32 @@ static Module module_for(ftr_ctx* c, const ftrh::Prep& pr, const Variant& v) {
   return m;
 }
 
+/* Jump polynomials of a launch geometry: lane jumps S*per*2^j (j < lb),
+ * then, for base-16 digits k < ndig of the block index, the jumps of
+ * d 16^k blocks (d = 1..15); cached per context. */
+static const std::vector<uint64_t>& jump_polys(ftr_ctx* c, uint64_t S, uint64_t per, int lb, int ndig) {
+  const auto key = std::make_tuple(S, per, lb, ndig);
+  auto it = c->jump_cache.find(key);
+  if (it != c->jump_cache.end()) return it->second;
+  std::vector<uint64_t> polys(4 * ((size_t)lb + 15 * (size_t)ndig));
+  const unsigned __int128 dl = (unsigned __int128)S * per;
+  ftrh::Poly q = ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64));
+  for (int j = 0; j < lb; j++) {
+    memcpy(&polys[4 * j], q.w, 32);
+    q = ftrh::xo_poly_square(q);
+  }
+  /* q is now the jump polynomial for one block (S * per * block) */
+  for (int k = 0; k < ndig; k++) {
+    ftrh::Poly r = q;
+    for (int d = 1; d <= 15; d++) {
+      memcpy(&polys[4 * ((size_t)lb + 15 * k + d - 1)], r.w, 32);
+      if (d < 15) r = ftrh::xo_poly_mul(r, q);
+    }
+    for (int sq = 0; sq < 4; sq++) q = ftrh::xo_poly_square(q); /* 16^(k+1) blocks */
+  }
+  return c->jump_cache.emplace(key, std::move(polys)).first->second;
+}
+
 Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
                       int nq, hipFunction_t occ_fn) {
   ftr_ctx* c = p->ctx;
@@ -394,33 +420,39 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
   const uint64_t per = 2 * (uint64_t)E - 1;
   int lb = 0;
   while ((1 << lb) < block) lb++;
-  std::vector<uint64_t> states(4 * nblocks), polys(4 * (size_t)lb);
-  uint64_t st[4];
-  ftrh::xo_seed(seed, st);
+  int ndig = 0;
+  while (ndig < 16 && ((uint64_t)1 << (4 * ndig)) < nblocks) ndig++;
+  const std::vector<uint64_t>& polys = jump_polys(c, S, per, lb, ndig);
+  /* the stream state at sample `first`; block b starts b blocks later
+   * (ftrj_block_states on the device) */
+  std::vector<uint64_t> base(4);
+  ftrh::xo_seed(seed, base.data());
   {
     const unsigned __int128 d0 = (unsigned __int128)first * per;
-    if (d0) ftrh::xo_apply(ftrh::xo_jump_poly((uint64_t)d0, (uint64_t)(d0 >> 64)), st);
+    if (d0) ftrh::xo_apply(ftrh::xo_jump_poly((uint64_t)d0, (uint64_t)(d0 >> 64)), base.data());
   }
-  const unsigned __int128 dl = (unsigned __int128)S * per;
-  ftrh::Poly q = ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64));
-  for (int j = 0; j < lb; j++) {
-    memcpy(&polys[4 * j], q.w, 32);
-    q = ftrh::xo_poly_square(q);
-  }
-  /* q is now the jump polynomial for one block (S * per * block) */
-  for (uint64_t b = 0; b < nblocks; b++) {
-    memcpy(&states[4 * b], st, 32);
-    if (b + 1 < nblocks) ftrh::xo_apply(q, st);
-  }
-  const size_t bytes_states = states.size() * 8, bytes_polys = std::max<size_t>(polys.size() * 8, 32);
+  const size_t bytes_states = (size_t)nblocks * 32, bytes_polys = polys.size() * 8 + 32;
   const size_t bytes_part = (size_t)nblocks * nq * 8;
-  ensure_scratch(c, bytes_states + bytes_polys + bytes_part + 256);
-  char* base = (char*)c->d_scratch;
-  G.d_states = (uint64_t*)base;
-  G.d_polys = (uint64_t*)(base + ((bytes_states + 255) & ~(size_t)255));
+  ensure_scratch(c, bytes_states + bytes_polys + bytes_part + 512);
+  char* d = (char*)c->d_scratch;
+  G.d_states = (uint64_t*)d;
+  G.d_polys = (uint64_t*)(d + ((bytes_states + 255) & ~(size_t)255));
   G.d_partials = (double*)((char*)G.d_polys + ((bytes_polys + 255) & ~(size_t)255));
-  HIPCHK(hipMemcpyHtoDAsync(G.d_states, states.data(), bytes_states, c->stream));
-  if (!polys.empty()) HIPCHK(hipMemcpyHtoDAsync(G.d_polys, polys.data(), polys.size() * 8, c->stream));
+  uint64_t* d_base = G.d_polys + polys.size();
+  if (!polys.empty()) HIPCHK(hipMemcpyHtoDAsync(G.d_polys, (void*)polys.data(), polys.size() * 8, c->stream));
+  HIPCHK(hipMemcpyHtoDAsync(d_base, base.data(), 32, c->stream));
+  if (!c->f_block_states) {
+    hipModule_t mod = load_module_raw(c, ftr_embedded_jtable_source);
+    HIPCHK(hipModuleGetFunction(&c->f_block_states, mod, "ftrj_block_states"));
+  }
+  {
+    const uint64_t* bp = G.d_polys + 4 * lb;
+    int nb = ndig;
+    uint64_t nbl = nblocks;
+    void* args[] = {&d_base, &bp, &nb, &nbl, &G.d_states};
+    HIPCHK(hipModuleLaunchKernel(c->f_block_states, (unsigned)((nblocks + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream,
+                                 args, nullptr));
+  }
   return G;
 }
 

@@ -113,6 +113,7 @@ static Poly mulmod(const Poly& a, const Poly& b) {
 }
 
 Poly xo_poly_square(const Poly& a) { return mulmod(a, a); }
+Poly xo_poly_mul(const Poly& a, const Poly& b) { return mulmod(a, b); }
 
 Poly xo_jump_poly(uint64_t lo, uint64_t hi) {
   Poly res, base;

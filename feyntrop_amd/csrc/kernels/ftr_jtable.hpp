@@ -255,6 +255,54 @@ __device__ inline void jt_level_body(const JtParams* __restrict__ prm, int n, ui
 
 }  // namespace ftrj
 
+/* Stream start states of the blocks of a launch: state of block b = the
+ * base state jumped by b blocks.  b is written in base 16; digit k with
+ * value d > 0 applies the jump polynomial of d 16^k blocks, polys[(15 k +
+ * d - 1) 4 ...] (a jump by polynomial q maps the state s to q(M) s, M the
+ * xoshiro256+ transition over GF(2); jumps commute, so any decomposition
+ * gives the same state).  One thread per block. */
+__device__ inline void xo_apply_dev(uint64_t (&s)[4], const uint64_t* q) {
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (int w = 0; w < 4; w++) {
+    const uint64_t bits = q[w];
+    for (int b = 0; b < 64; b++) {
+      if ((bits >> b) & 1) {
+        acc[0] ^= s[0];
+        acc[1] ^= s[1];
+        acc[2] ^= s[2];
+        acc[3] ^= s[3];
+      }
+      const uint64_t t = s[1] << 17;
+      s[2] ^= s[0];
+      s[3] ^= s[1];
+      s[1] ^= s[2];
+      s[0] ^= s[3];
+      s[2] ^= t;
+      s[3] = (s[3] << 45) | (s[3] >> 19);
+    }
+  }
+  s[0] = acc[0];
+  s[1] = acc[1];
+  s[2] = acc[2];
+  s[3] = acc[3];
+}
+
+extern "C" __global__ void __launch_bounds__(256) ftrj_block_states(const uint64_t* base, const uint64_t* polys,
+                                                                     int ndig, uint64_t nblocks, uint64_t* out) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  uint64_t s[4] = {base[0], base[1], base[2], base[3]};
+  for (int k = 0; k < ndig; k++) {
+    const int d = (int)((b >> (4 * k)) & 15);
+    if (d) xo_apply_dev(s, polys + 4 * (15 * k + d - 1));
+  }
+  out[4 * b + 0] = s[0];
+  out[4 * b + 1] = s[1];
+  out[4 * b + 2] = s[2];
+  out[4 * b + 3] = s[3];
+}
+
+
 #define FTR_JT_KERNEL(VB)                                                                                       \
   extern "C" __global__ void __launch_bounds__(64)                                                              \
       ftrj_level_##VB(const ftrj::JtParams* prm, int n, uint64_t count, const uint64_t* binom, const double* Xs, \
