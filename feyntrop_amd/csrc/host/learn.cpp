@@ -37,10 +37,13 @@ constexpr int HIST_E = 10;         /* histogram in LDS up to this E (ftr_device.
 constexpr size_t hist_len(int E) { return ((size_t)E + 1) * ((size_t)1 << E) + 2; }
 constexpr int LEARN_BLOCK = 256;  /* one wave per SIMD per block: the full register budget per lane */
 constexpr double SMAX = 0.6;       /* |sigma| bound: a = r e^sigma < 2r keeps p/q of finite variance */
+constexpr int MAX_SPLITS = 64;     /* split-K blocks of a weight-gradient GEMM */
+constexpr int MAX_CHUNKS = 256;    /* row chunks of a bias-gradient column sum */
+constexpr size_t WS_FLOATS = (size_t)MAX_SPLITS * H * H; /* their workspace */
 
 struct MlpModule {
   hipModule_t mod = nullptr;
-  hipFunction_t gbb, gfb, silu_bwd, colsum, adam, fill, fill_f, to_bf16, tables, tgrad;
+  hipFunction_t gbb, gfb, silu_bwd, colsum, split_sum, adam, fill, fill_f, to_bf16, tables, tgrad;
 };
 
 MlpModule load_mlp(ftr_ctx* c) {
@@ -50,6 +53,7 @@ MlpModule load_mlp(ftr_ctx* c) {
   HIPCHK(hipModuleGetFunction(&m.gfb, m.mod, "mlp_gemm_fb"));
   HIPCHK(hipModuleGetFunction(&m.silu_bwd, m.mod, "mlp_silu_bwd"));
   HIPCHK(hipModuleGetFunction(&m.colsum, m.mod, "mlp_colsum"));
+  HIPCHK(hipModuleGetFunction(&m.split_sum, m.mod, "mlp_split_sum"));
   HIPCHK(hipModuleGetFunction(&m.adam, m.mod, "mlp_adam"));
   HIPCHK(hipModuleGetFunction(&m.fill, m.mod, "mlp_fill"));
   HIPCHK(hipModuleGetFunction(&m.fill_f, m.mod, "mlp_fill_f"));
@@ -77,9 +81,11 @@ struct ftr_learner {
   uint32_t NT = 0;
   int M = 0; /* network rows: max(32, 2^E) */
   MlpModule mlp;
-  hipFunction_t f_lev = nullptr, f_lgr = nullptr, f_lsm = nullptr, f_acc = nullptr, f_hst = nullptr,
-                f_hrd = nullptr;
-  double* d_part = nullptr; /* per-block histograms (E <= HIST_E) */
+  hipFunction_t f_lev = nullptr, f_lgr = nullptr, f_lsm = nullptr, f_hmx = nullptr, f_acc = nullptr,
+                f_hst = nullptr, f_hrd = nullptr;
+  ftr::fixacc* d_part = nullptr;     /* fixed-point statistics: per block (E <= HIST_E) or one */
+  unsigned long long* d_gmax = nullptr; /* the step's largest |statistic| (fixed-point scale) */
+  float* d_ws = nullptr;             /* split-K and column-sum partials */
   int hist_blocks = 0;
   double* d_gbuf = nullptr; /* training record (see ftr_abi.hpp LearnLaunch) */
   uint64_t* d_gpath = nullptr;
@@ -113,21 +119,32 @@ void launch(hipFunction_t f, unsigned gx, unsigned gy, unsigned block, hipStream
   HIPCHK(hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, 0, s, args, nullptr));
 }
 
+/* out[i] = sum over z < splits of ws[z n + i] (fixed order) */
+void split_sum(ftr_ctx* c, ftr_learner* l, float* out, uint64_t n, int splits) {
+  void* args[] = {&l->d_ws, &out, &n, &splits};
+  launch(l->mlp.split_sum, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
+}
+
 /* C = act(A B + bias), operands addressed by strides (see ftr_mlp.hpp) */
 void gemm(ftr_ctx* c, ftr_learner* l, bool a_float, const void* A, uint64_t sai, uint64_t sak, const uint16_t* B,
           uint64_t sbk, uint64_t sbj, const float* bias, float* C, uint64_t ldc, uint16_t* Cb, uint64_t ldcb, int M,
-          int N, int K, int act, bool c_zeroed = false) {
+          int N, int K, int act, bool split_k = false) {
   if (M % 32 || N % 32 || K % 8) throw Error(FTR_ERR_INVALID, "network GEMM shape not tiled");
-  /* long reductions (weight gradients over all states) split K over blocks
-   * that add into C atomically when the caller has zeroed C */
+  /* long reductions (weight gradients over all states) split K over at
+   * most MAX_SPLITS blocks; their partial products go to the workspace and
+   * are added in a fixed order (reproducible, no atomics) */
   int splits = 1, kc = K;
-  if (c_zeroed && K >= 256 && !Cb && !bias && !act) {
-    kc = 64;
+  if (split_k && K >= 256 && !Cb && !bias && !act) {
+    if (ldc != (uint64_t)N || (size_t)MAX_SPLITS * M * N > WS_FLOATS)
+      throw Error(FTR_ERR_INVALID, "network GEMM split does not fit the workspace");
+    kc = std::max(64, (((K + MAX_SPLITS - 1) / MAX_SPLITS) + 7) / 8 * 8);
     splits = (K + kc - 1) / kc;
   }
-  void* args[] = {(void*)&A, &sai, &sak, (void*)&B, &sbk, &sbj, (void*)&bias, &C, &ldc, &Cb, &ldcb, &K, &act, &kc};
+  float* dst = splits > 1 ? l->d_ws : C;
+  void* args[] = {(void*)&A, &sai, &sak, (void*)&B, &sbk, &sbj, (void*)&bias, &dst, &ldc, &Cb, &ldcb, &K, &act, &kc};
   HIPCHK(hipModuleLaunchKernel(a_float ? l->mlp.gfb : l->mlp.gbb, (unsigned)(N / 32), (unsigned)(M / 32),
                                (unsigned)splits, 64, 1, 1, 0, c->stream, args, nullptr));
+  if (splits > 1) split_sum(c, l, C, (uint64_t)M * N, splits);
 }
 
 void fill_d(ftr_ctx* c, ftr_learner* l, double* p, uint64_t n, double v) {
@@ -157,11 +174,15 @@ void forward(ftr_ctx* c, ftr_learner* l) {
   gemm(c, l, false, l->d_H3, H, 1, W + OFF_W4, 1, H, P + OFF_B4, l->d_Y, NOUT, nullptr, 0, M, NOUT, H, 0);
 }
 
-/* db += column sums of dZ (db zeroed beforehand) */
+/* db = column sums of dZ: sums over row chunks, then the chunks added in a
+ * fixed order */
 void colsum(ftr_ctx* c, ftr_learner* l, const float* dZ, float* db, int rows, int cols) {
-  int chunk = 32;
-  void* args[] = {(void*)&dZ, &db, &rows, &cols, &chunk};
-  launch(l->mlp.colsum, (unsigned)((cols + 63) / 64), (unsigned)((rows + chunk - 1) / chunk), 64, c->stream, args);
+  int chunk = std::max(32, (rows + MAX_CHUNKS - 1) / MAX_CHUNKS);
+  const int nchunks = (rows + chunk - 1) / chunk;
+  if ((size_t)nchunks * cols > WS_FLOATS) throw Error(FTR_ERR_INVALID, "network column sum does not fit the workspace");
+  void* args[] = {(void*)&dZ, &l->d_ws, &rows, &cols, &chunk};
+  launch(l->mlp.colsum, (unsigned)((cols + 63) / 64), (unsigned)nchunks, 64, c->stream, args);
+  split_sum(c, l, db, (uint64_t)cols, nchunks);
 }
 
 void silu_bwd(ftr_ctx* c, ftr_learner* l, const float* dH, const float* Z, float* dZ, uint64_t n) {
@@ -174,7 +195,6 @@ void backward(ftr_ctx* c, ftr_learner* l) {
   const uint16_t* W = l->d_parb;
   float* G = l->d_grad;
   const int M = l->M;
-  fill_f(c, l, G, NPAR, 0.0f); /* split-K GEMMs and column sums accumulate */
   /* layer 4: dW4 = dY^T H3, db4 = colsum dY, dH3 = dY W4 */
   gemm(c, l, true, l->d_dY, 1, NOUT, l->d_H3, H, 1, nullptr, G + OFF_W4, H, nullptr, 0, NOUT, H, M, 0, true);
   colsum(c, l, l->d_dY, G + OFF_B4, M, NOUT);
@@ -233,6 +253,7 @@ ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
   L.gC = l->d_gC;
   L.gS = l->d_gS;
   L.gsum = l->d_gsum;
+  L.gmax = l->d_gmax;
   L.gbuf = l->d_gbuf;
   L.gpath = l->d_gpath;
   L.objective = l->objective;
@@ -298,6 +319,7 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     HIPCHK(hipModuleGetFunction(&l->f_lev, gm, "ftrl_lev"));
     HIPCHK(hipModuleGetFunction(&l->f_lgr, gm, "ftrl_lgr"));
     HIPCHK(hipModuleGetFunction(&l->f_lsm, gm, "ftrl_lsm"));
+    HIPCHK(hipModuleGetFunction(&l->f_hmx, gm, "ftrl_hmx"));
     HIPCHK(hipModuleGetFunction(&l->f_acc, gm, "ftrl_acc"));
     HIPCHK(hipModuleGetFunction(&l->f_hst, gm, "ftrl_hst"));
     HIPCHK(hipModuleGetFunction(&l->f_hrd, gm, "ftrl_hrd"));
@@ -307,8 +329,13 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
       if (hb > 64 * 1024)
         HIPCHK(hipFuncSetAttribute((const void*)l->f_hst, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hb));
       l->hist_blocks = c->prop.multiProcessorCount;
-      l->d_part = dalloc<double>(l, (size_t)l->hist_blocks * hist_len(E));
+      l->d_part = dalloc<ftr::fixacc>(l, (size_t)l->hist_blocks * hist_len(E));
+    } else {
+      l->hist_blocks = 1;
+      l->d_part = dalloc<ftr::fixacc>(l, hist_len(E));
     }
+    l->d_gmax = dalloc<unsigned long long>(l, 1);
+    l->d_ws = dalloc<float>(l, WS_FLOATS);
     if (l->lds > 64 * 1024)
       for (hipFunction_t f : {l->f_lev, l->f_lgr, l->f_lsm})
         (void)hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l->lds);
@@ -481,31 +508,32 @@ int ftr_learner_train(ftr_ctx* c, ftr_learner* l, int steps, uint64_t samples, u
     for (int st = 0; st < steps; st++) {
       forward(c, l);
       build_tables(c, l);
-      if (E > HIST_E) { /* the atomic accumulation adds into them */
-        fill_d(c, l, l->d_gC, (uint64_t)l->NT * l->E, 0.0);
-        fill_d(c, l, l->d_gS, l->NT, 0.0);
-        fill_d(c, l, l->d_gsum, 2, 0.0);
-      }
+      /* fixed-point statistics: the scale's maximum and (E > HIST_E) the
+       * one accumulator start at zero (zero bits) */
+      fill_d(c, l, reinterpret_cast<double*>(l->d_gmax), 1, 0.0);
+      if (E > HIST_E) fill_d(c, l, reinterpret_cast<double*>(l->d_part), hist_len(E), 0.0);
       Geometry G = setup_stream(p, 0, samples, seed + (uint64_t)st, 0, l->var.block, NQ, l->f_lgr);
       ftr::LearnLaunch L = learn_launch(l, G, samples);
       void* args[] = {&L};
       HIPCHK(hipEventRecord(c->ev0, c->stream));
       HIPCHK(hipModuleLaunchKernel(l->f_lgr, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
                                    c->stream, args, nullptr));
-      if (E <= HIST_E) {
+      {
         uint64_t n = samples;
+        void* margs[] = {&L, &n};
+        HIPCHK(hipModuleLaunchKernel(l->f_hmx, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream,
+                                     margs, nullptr));
         void* hargs[] = {&L, &n, &l->d_part};
-        HIPCHK(hipModuleLaunchKernel(l->f_hst, (unsigned)l->hist_blocks, 1, 1, 1024, 1, 1,
-                                     (unsigned)(hist_len(E) * 8), c->stream, hargs, nullptr));
+        if (E <= HIST_E)
+          HIPCHK(hipModuleLaunchKernel(l->f_hst, (unsigned)l->hist_blocks, 1, 1, 1024, 1, 1,
+                                       (unsigned)(hist_len(E) * 8), c->stream, hargs, nullptr));
+        else
+          HIPCHK(hipModuleLaunchKernel(l->f_acc, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1,
+                                       (unsigned)acc_lds, c->stream, hargs, nullptr));
         int nb = l->hist_blocks;
-        void* rargs[] = {&L, &l->d_part, &nb};
+        void* rargs[] = {&L, &l->d_part, &nb, &n};
         HIPCHK(hipModuleLaunchKernel(l->f_hrd, (unsigned)((hist_len(E) + 255) / 256), 1, 1, 256, 1, 1, 0,
                                      c->stream, rargs, nullptr));
-      } else {
-        uint64_t n = samples;
-        void* aargs[] = {&L, &n};
-        HIPCHK(hipModuleLaunchKernel(l->f_acc, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1,
-                                     (unsigned)acc_lds, c->stream, aargs, nullptr));
       }
       HIPCHK(hipEventRecord(c->ev1, c->stream));
       fill_f(c, l, l->d_dY, (uint64_t)l->M * NOUT, 0.0f);

@@ -81,9 +81,10 @@ struct BatchItem {
  *   lia[g]        1 / a(g), a(g) = r(g) exp(sigma(g)): radial exponent after
  *                 arriving at g (plain sampling: a = r)
  *   lrad[g]       log(r/a) = -sigma(g);   ldiff[g]  r(g) - a(g)
- * Training accumulators (doubles): gC[g*E+e] += c for every visit of g that
- * cut e; gS[g] += c (1 + a(g) log Y) for the radius drawn after arriving at
- * g; gsum[0] += c, gsum[1] += 1, with c = weight * |R_0|. */
+ * Training statistics (doubles, sums over a step's samples, accumulated in
+ * fixed point): gC[g*E+e] = sum of c over visits of g that cut e; gS[g] =
+ * sum of c (1 + a(g) log Y) over the radii drawn after arriving at g;
+ * gsum = (sum of c, number of samples), with c = weight * |R_0|. */
 /* Per-state record of the learned tables, one 64-byte load per visited state:
  * lZ(g), A(g) = lZ(g)/J(g), the radial parameters lia, lrad, ldiff and the
  * state's mm | L<<1 flags. */
@@ -92,6 +93,9 @@ struct alignas(64) LQuad {
   uint32_t mmL, pad0;
   double pad1, pad2;
 };
+
+/* Fixed-point accumulator of the training statistics (two's complement). */
+typedef unsigned long long fixacc;
 
 struct LearnLaunch {
   Launch a;
@@ -106,6 +110,7 @@ struct LearnLaunch {
   double* gC;
   double* gS;
   double* gsum;
+  unsigned long long* gmax; /* largest |statistic| of the step, double bits */
   double* gbuf;      /* training record: c (N), then lY_j (N each, j < E-1) */
   int objective;     /* 0: forward KL, c = w|f|;  1: variance, c = (w|f|)^2 */
   uint64_t* gpath;   /* training record: cut order per sample */

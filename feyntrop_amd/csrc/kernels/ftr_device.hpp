@@ -1010,84 +1010,155 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   }
 }
 
-/* Forward-KL statistics of the recorded samples (one thread per sample):
- * gC[g E + e] += c for every visit of state g that cut e, gS[g] +=
- * c (1 + a(g) lY) for the radius drawn at g, gsum += (c, 1).  The states of
- * the first three levels (full graph, one and two edges removed) are shared
- * by many samples: their sums go through LDS (fixed indices: removed edge,
- * removed pair) and reach global memory once per block. */
+/* Forward-KL statistics of the recorded samples:
+ *   C(g, e) = sum of c over visits of state g that cut e,
+ *   S(g)    = sum of c (1 + a(g) lY) over the radii drawn on arriving at g,
+ *   gsum    = (sum of c, number of samples).
+ * They are kept in fixed point: every value is rounded to an integer
+ * multiple of 2^-k and added in 64-bit integers, so the sums do not depend
+ * on the order in which threads add them and training is reproducible bit
+ * for bit.  k is fixed per step from the largest |value| M over the step's N
+ * samples (learn_max_body) so that N M 2^k < 2^62: no sum can overflow.  The
+ * rounding is relative to the largest value (2^-62 N M at most); the
+ * gradients are formed in FP32.  Non-finite values count as 0.
+ * Layout of an accumulator (learn_hist_len entries): C at g E + e, S at
+ * 2^E E + g, then the two sums. */
 __host__ __device__ constexpr int learn_pairs(int E) { return E * (E - 1) / 2; }
 __host__ __device__ constexpr size_t learn_acc_lds(int E) {
   return (size_t)(2 + 2 * E + E * E + learn_pairs(E) * (1 + E)) * 8;
+}
+__host__ __device__ constexpr size_t learn_hist_len(int E) {
+  return ((size_t)E + 1) * ((size_t)1 << E) + 2;
 }
 __device__ inline int pair_index(int a, int b) {
   const int hi = a > b ? a : b, lo = a > b ? b : a;
   return hi * (hi - 1) / 2 + lo;
 }
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
 
+/* c (1 + a(cg) lY_j): the radial statistic of step j of sample i */
+__device__ inline double learn_radial(const LearnLaunch& L, double c, uint64_t N, uint64_t i, int j, uint32_t cg) {
+  return c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[cg]);
+}
+
+__device__ inline bool finite_d(double v) { return __builtin_isfinite(v); }
+
+/* 2^k with N M 2^k < 2^62 (M = *gmax); exponents from the bit patterns
+ * (frexp convention: x < 2^e), k clamped to [-1000, 1000]. */
+__device__ inline double learn_fix_scale(const LearnLaunch& L, uint64_t N) {
+  const long long mb = (long long)*L.gmax;
+  if (mb <= 0) return 1.0;
+  const int em = (int)((mb >> 52) & 0x7ff) - 1022;
+  const int en = (int)((__double_as_longlong((double)N) >> 52) & 0x7ff) - 1022;
+  int k = 62 - em - en;
+  k = k > 1000 ? 1000 : (k < -1000 ? -1000 : k);
+  return __longlong_as_double((long long)(k + 1023) << 52);
+}
+
+__device__ inline fixacc to_fix(double v, double scale) { return (fixacc)(long long)rint(v * scale); }
+
+/* *gmax = max |value| over the step's samples, as the bits of a
+ * non-negative double (whose order is that of the integers). */
 template <int E>
-__device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N) {
+__device__ inline void learn_max_body(const LearnLaunch& L, uint64_t N) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (i < N) {
+    const double c = L.gbuf[i];
+    if (c != 0.0 && finite_d(c)) {
+      m = fabs(c);
+      const uint64_t path = L.gpath[i];
+      uint32_t g = (1u << E) - 1u;
+      for (int j = 0; j < E - 1; j++) {
+        const uint32_t cg = g ^ (1u << (int)((path >> (4 * j)) & 15u));
+        const double sv = learn_radial(L, c, N, i, j, cg);
+        if (finite_d(sv)) m = fmax(m, fabs(sv));
+        g = cg;
+      }
+    }
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax((unsigned long long*)L.gmax, (unsigned long long)__double_as_longlong(m));
+}
+
+/* E > FTR_LEARN_LDS_E: one thread per sample, adding into one accumulator
+ * `acc` (zeroed beforehand).  The states of the first three levels (full
+ * graph, one and two edges removed) are shared by many samples: their sums
+ * go through LDS (fixed indices: removed edge, removed pair) and reach
+ * global memory once per block. */
+template <int E>
+__device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N, fixacc* acc) {
   extern __shared__ __align__(16) char lds[];
-  double* sh = reinterpret_cast<double*>(lds);
+  fixacc* sh = reinterpret_cast<fixacc*>(lds);
   constexpr int NP = learn_pairs(E);
-  double* s_sum = sh;                  /* 2 */
-  double* s_c0 = s_sum + 2;            /* E: cut e0 from the full graph */
-  double* s_s1 = s_c0 + E;             /* E: radius at full \ e0 */
-  double* s_c1 = s_s1 + E;             /* E x E: cut e1 from full \ e0 */
-  double* s_s2 = s_c1 + E * E;         /* NP: radius at full \ {e0, e1} */
-  double* s_c2 = s_s2 + NP;            /* NP x E: cut e2 from full \ {e0, e1} */
+  constexpr uint32_t NT = 1u << E;
+  fixacc* s_sum = sh;                  /* 2 */
+  fixacc* s_c0 = s_sum + 2;            /* E: cut e0 from the full graph */
+  fixacc* s_s1 = s_c0 + E;             /* E: radius at full \ e0 */
+  fixacc* s_c1 = s_s1 + E;             /* E x E: cut e1 from full \ e0 */
+  fixacc* s_s2 = s_c1 + E * E;         /* NP: radius at full \ {e0, e1} */
+  fixacc* s_c2 = s_s2 + NP;            /* NP x E: cut e2 from full \ {e0, e1} */
   constexpr int NSH = 2 + 2 * E + E * E + NP * (1 + E);
-  for (int q = threadIdx.x; q < NSH; q += blockDim.x) sh[q] = 0.0;
+  fixacc* aC = acc;
+  fixacc* aS = acc + (size_t)NT * E;
+  fixacc* asum = aS + NT;
+  for (int q = threadIdx.x; q < NSH; q += blockDim.x) sh[q] = 0;
   __syncthreads();
+  const double scale = learn_fix_scale(L, N);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N) {
     const double c = L.gbuf[i];
-    atomicAdd(&s_sum[1], 1.0);
-    if (c != 0.0) {
+    atomicAdd(&s_sum[1], (fixacc)1);
+    if (c != 0.0 && finite_d(c)) {
+      const fixacc qc = to_fix(c, scale);
       const uint64_t path = L.gpath[i];
-      atomicAdd(&s_sum[0], c);
-      uint32_t g = (1u << E) - 1u;
+      atomicAdd(&s_sum[0], qc);
+      uint32_t g = NT - 1u;
       int e0 = 0, e1 = 0;
       for (int j = 0; j < E; j++) {
         const int e = (int)((path >> (4 * j)) & 15u);
         const uint32_t cg = g ^ (1u << e);
-        double sv = 0.0;
-        if (j < E - 1) sv = c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[cg]);
+        fixacc qs = 0;
+        if (j < E - 1) {
+          const double sv = learn_radial(L, c, N, i, j, cg);
+          qs = finite_d(sv) ? to_fix(sv, scale) : (fixacc)0;
+        }
         if (j == 0) {
           e0 = e;
-          atomicAdd(&s_c0[e], c);
-          if (j < E - 1) atomicAdd(&s_s1[e], sv);
+          atomicAdd(&s_c0[e], qc);
+          atomicAdd(&s_s1[e], qs);
         } else if (j == 1) {
           e1 = e;
-          atomicAdd(&s_c1[e0 * E + e], c);
-          if (j < E - 1) atomicAdd(&s_s2[pair_index(e0, e1)], sv);
-        } else if (j == 2) {
-          atomicAdd(&s_c2[pair_index(e0, e1) * E + e], c);
-          if (j < E - 1) unsafeAtomicAdd(&L.gS[cg], sv);
+          atomicAdd(&s_c1[e0 * E + e], qc);
+          atomicAdd(&s_s2[pair_index(e0, e1)], qs);
         } else {
-          unsafeAtomicAdd(&L.gC[(size_t)g * E + e], c);
-          if (j < E - 1) unsafeAtomicAdd(&L.gS[cg], sv);
+          if (j == 2) atomicAdd(&s_c2[pair_index(e0, e1) * E + e], qc);
+          else atomicAdd(&aC[(size_t)g * E + e], qc);
+          if (j < E - 1) atomicAdd(&aS[cg], qs);
         }
         g = cg;
       }
     }
   }
   __syncthreads();
-  const uint32_t full = (1u << E) - 1u;
+  const uint32_t full = NT - 1u;
   for (int q = threadIdx.x; q < NSH; q += blockDim.x) {
-    const double v = sh[q];
-    if (v == 0.0) continue;
+    const fixacc v = sh[q];
+    if (v == 0) continue;
     if (q < 2) {
-      unsafeAtomicAdd(&L.gsum[q], v);
+      atomicAdd(&asum[q], v);
     } else if (q < 2 + E) {
-      const int e = q - 2;
-      unsafeAtomicAdd(&L.gC[(size_t)full * E + e], v);
+      atomicAdd(&aC[(size_t)full * E + (q - 2)], v);
     } else if (q < 2 + 2 * E) {
-      const int e = q - 2 - E;
-      unsafeAtomicAdd(&L.gS[full ^ (1u << e)], v);
+      atomicAdd(&aS[full ^ (1u << (q - 2 - E))], v);
     } else if (q < 2 + 2 * E + E * E) {
       const int k = q - 2 - 2 * E, e0 = k / E, e = k % E;
-      unsafeAtomicAdd(&L.gC[(size_t)(full ^ (1u << e0)) * E + e], v);
+      atomicAdd(&aC[(size_t)(full ^ (1u << e0)) * E + e], v);
     } else {
       const int k = q - 2 - 2 * E - E * E;
       int pidx, e = -1;
@@ -1101,80 +1172,82 @@ __device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N) {
       while ((hi + 1) * hi / 2 <= pidx) hi++;
       const int lo = pidx - hi * (hi - 1) / 2;
       const uint32_t gg = full ^ (1u << hi) ^ (1u << lo);
-      if (e < 0) unsafeAtomicAdd(&L.gS[gg], v);
-      else unsafeAtomicAdd(&L.gC[(size_t)gg * E + e], v);
+      if (e < 0) atomicAdd(&aS[gg], v);
+      else atomicAdd(&aC[(size_t)gg * E + e], v);
     }
   }
 }
 
-/* Forward-KL statistics for E <= FTR_LEARN_LDS_E: every block holds the
- * whole histogram (gC, gS, gsum) in LDS, walks a strided share of the
- * recorded samples with LDS atomics and stores its histogram to its own
- * slice of `part` (no global atomics); learn_reduce_body then sums the
- * slices in block order, so the statistics are deterministic. */
+/* E <= FTR_LEARN_LDS_E: every block holds a whole accumulator in LDS, walks
+ * a strided share of the recorded samples and stores its accumulator to its
+ * own slice of `part`; learn_reduce_body sums the slices. */
 #define FTR_LEARN_LDS_E 10
-__host__ __device__ constexpr size_t learn_hist_len(int E) {
-  return ((size_t)E + 1) * ((size_t)1 << E) + 2;
-}
 
 template <int E>
-__device__ inline void learn_hist_body(const LearnLaunch& L, uint64_t N, double* part) {
+__device__ inline void learn_hist_body(const LearnLaunch& L, uint64_t N, fixacc* part) {
   if constexpr (E <= FTR_LEARN_LDS_E) {
     extern __shared__ __align__(16) char lds[];
-    double* h = reinterpret_cast<double*>(lds);
+    fixacc* h = reinterpret_cast<fixacc*>(lds);
     constexpr uint32_t NT = 1u << E;
     constexpr size_t HL = learn_hist_len(E);
-    double* hC = h;                 /* NT x E */
-    double* hS = h + (size_t)NT * E; /* NT */
-    double* hsum = hS + NT;         /* 2 */
-    for (size_t q = threadIdx.x; q < HL; q += blockDim.x) h[q] = 0.0;
+    fixacc* hC = h;                  /* NT x E */
+    fixacc* hS = h + (size_t)NT * E; /* NT */
+    fixacc* hsum = hS + NT;          /* 2 */
+    for (size_t q = threadIdx.x; q < HL; q += blockDim.x) h[q] = 0;
     __syncthreads();
-    double c_sum = 0.0, n_sum = 0.0;
+    const double scale = learn_fix_scale(L, N);
+    fixacc c_sum = 0, n_sum = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
       const double c = L.gbuf[i];
-      n_sum += 1.0;
-      if (c == 0.0) continue;
-      c_sum += c;
+      n_sum += 1;
+      if (c == 0.0 || !finite_d(c)) continue;
+      const fixacc qc = to_fix(c, scale);
+      c_sum += qc;
       const uint64_t path = L.gpath[i];
       uint32_t g = NT - 1u;
       for (int j = 0; j < E; j++) {
         const int e = (int)((path >> (4 * j)) & 15u);
         const uint32_t cg = g ^ (1u << e);
-        atomicAdd(&hC[(size_t)g * E + e], c);
-        if (j < E - 1) atomicAdd(&hS[cg], c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[cg]));
+        atomicAdd(&hC[(size_t)g * E + e], qc);
+        if (j < E - 1) {
+          const double sv = learn_radial(L, c, N, i, j, cg);
+          if (finite_d(sv)) atomicAdd(&hS[cg], to_fix(sv, scale));
+        }
         g = cg;
       }
     }
-    c_sum = wave_sum(c_sum);
-    n_sum = wave_sum(n_sum);
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&hsum[0], c_sum);
-      atomicAdd(&hsum[1], n_sum);
-    }
+    atomicAdd(&hsum[0], c_sum);
+    atomicAdd(&hsum[1], n_sum);
     __syncthreads();
-    double* out = part + (size_t)blockIdx.x * HL;
+    fixacc* out = part + (size_t)blockIdx.x * HL;
     for (size_t q = threadIdx.x; q < HL; q += blockDim.x) out[q] = h[q];
   }
 }
 
-/* gC, gS, gsum = sums of the nblk block histograms (one thread per entry). */
+/* gC, gS, gsum from the nblk accumulators (one thread per entry): the
+ * integer sum, times 2^-k (the sample count unscaled). */
 template <int E>
-__device__ inline void learn_reduce_body(const LearnLaunch& L, const double* part, int nblk) {
+__device__ inline void learn_reduce_body(const LearnLaunch& L, const fixacc* part, int nblk, uint64_t N) {
   constexpr uint32_t NT = 1u << E;
   constexpr size_t HL = learn_hist_len(E);
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= HL) return;
-  double sb[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* 8 loads in flight, fixed order */
+  fixacc sb[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* 8 loads in flight */
   int b = 0;
   for (; b + 8 <= nblk; b += 8)
 #pragma unroll
     for (int t = 0; t < 8; t++) sb[t] += part[(size_t)(b + t) * HL + q];
   for (; b < nblk; b++) sb[0] += part[(size_t)b * HL + q];
-  const double s = ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
-  if (q < (size_t)NT * E) L.gC[q] = s;
-  else if (q < (size_t)NT * E + NT) L.gS[q - (size_t)NT * E] = s;
-  else L.gsum[q - (size_t)NT * E - NT] = s;
+  const fixacc s = ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
+  if (q == HL - 1) {
+    L.gsum[1] = (double)s;
+    return;
+  }
+  const double v = (double)(long long)s / learn_fix_scale(L, N);
+  if (q < (size_t)NT * E) L.gC[q] = v;
+  else if (q < (size_t)NT * E + NT) L.gS[q - (size_t)NT * E] = v;
+  else L.gsum[0] = v;
 }
 
 /* Learned sampler only (parity): X, the weight p/q and the cut order of
@@ -1227,16 +1300,21 @@ __device__ inline void batch_body(const BatchItem* items, int nitems) {
                                                                    double* w, uint64_t* path) {   \
     ftr::learn_sample_body<G, TMODE>(L, X, w, path);                                              \
   }                                                                                               \
-  extern "C" __global__ void __launch_bounds__(256) NAME##_acc(ftr::LearnLaunch L, uint64_t N) {    \
-    ftr::learn_acc_body<G::E>(L, N);                                                              \
+  extern "C" __global__ void __launch_bounds__(256) NAME##_hmx(ftr::LearnLaunch L, uint64_t N) {    \
+    ftr::learn_max_body<G::E>(L, N);                                                              \
+  }                                                                                               \
+  extern "C" __global__ void __launch_bounds__(256) NAME##_acc(ftr::LearnLaunch L, uint64_t N,      \
+                                                              ftr::fixacc* acc) {                 \
+    ftr::learn_acc_body<G::E>(L, N, acc);                                                         \
   }                                                                                               \
   extern "C" __global__ void __launch_bounds__(1024) NAME##_hst(ftr::LearnLaunch L, uint64_t N,     \
-                                                                double* part) {                   \
+                                                                ftr::fixacc* part) {              \
     ftr::learn_hist_body<G::E>(L, N, part);                                                       \
   }                                                                                               \
-  extern "C" __global__ void __launch_bounds__(256) NAME##_hrd(ftr::LearnLaunch L, const double* part, \
-                                                               int nblk) {                        \
-    ftr::learn_reduce_body<G::E>(L, part, nblk);                                                  \
+  extern "C" __global__ void __launch_bounds__(256) NAME##_hrd(ftr::LearnLaunch L,                  \
+                                                               const ftr::fixacc* part, int nblk, \
+                                                               uint64_t N) {                      \
+    ftr::learn_reduce_body<G::E>(L, part, nblk, N);                                               \
   }
 
 /* Instantiation macro used by the generated kernel units:

@@ -59,8 +59,9 @@ __device__ inline short ld_bf<float>(const float* p, size_t off) {
  * Writes the FP32 result to C (if non-null) and its bf16 rounding after the
  * activation to Cb (if non-null).  act: 0 identity, 1 SiLU.  One wave per
  * output tile; grid = (N/32, M/32, splits) blocks of 64 threads.  With
- * splits > 1 block z covers K rows [z kc, (z+1) kc) and adds its partial
- * product to C atomically (C zeroed beforehand; bias, act and Cb unused). */
+ * splits > 1 block z covers K rows [z kc, (z+1) kc) and stores its partial
+ * product to slice z of C (slices of M x ldc floats; bias, act and Cb
+ * unused); mlp_split_sum then adds the slices in a fixed order. */
 template <class AT, class BT>
 __device__ inline void gemm_tile(const AT* A, size_t sai, size_t sak, const BT* B, size_t sbk, size_t sbj,
                                  const float* bias, float* C, size_t ldc, unsigned short* Cb, size_t ldcb,
@@ -89,7 +90,7 @@ __device__ inline void gemm_tile(const AT* A, size_t sai, size_t sak, const BT* 
     const int i = ti * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
     const int j = bc;
     if (split) {
-      unsafeAtomicAdd(&C[(size_t)i * ldc + j], acc[v]);
+      C[(size_t)blockIdx.z * gridDim.y * 32 * ldc + (size_t)i * ldc + j] = acc[v];
       continue;
     }
     const float z = acc[v] + (bias ? bias[j] : 0.0f);
@@ -120,9 +121,10 @@ extern "C" __global__ void mlp_silu_bwd(const float* dH, const float* Z, float* 
   if (i < n) dZ[i] = dH[i] * ftrm::dsilu(Z[i]);
 }
 
-/* db[j] += sum_i dZ[i][j]  (rows x cols, row-major; db zeroed beforehand):
- * one thread per column and chunk of `chunk` rows (grid.y = row chunks). */
-extern "C" __global__ void mlp_colsum(const float* dZ, float* db, int rows, int cols, int chunk) {
+/* part[y][j] = sum of dZ[i][j] over the rows i of chunk y (rows x cols,
+ * row-major): one thread per column and chunk of `chunk` rows (grid.y =
+ * row chunks); mlp_split_sum adds the chunks. */
+extern "C" __global__ void mlp_colsum(const float* dZ, float* part, int rows, int cols, int chunk) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
   const int i0 = blockIdx.y * chunk;
@@ -133,7 +135,20 @@ extern "C" __global__ void mlp_colsum(const float* dZ, float* db, int rows, int 
 #pragma unroll
     for (int t = 0; t < 4; t++) s[t] += dZ[(size_t)(i + t) * cols + j];
   for (; i < i1; i++) s[0] += dZ[(size_t)i * cols + j];
-  unsafeAtomicAdd(&db[j], (s[0] + s[1]) + (s[2] + s[3]));
+  part[(size_t)blockIdx.y * cols + j] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+/* out[i] = sum over z < splits of ws[z n + i], in a fixed order. */
+extern "C" __global__ void mlp_split_sum(const float* ws, float* out, uint64_t n, int splits) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}; /* 4 loads in flight */
+  int z = 0;
+  for (; z + 4 <= splits; z += 4)
+#pragma unroll
+    for (int t = 0; t < 4; t++) s[t] += ws[(size_t)(z + t) * n + i];
+  for (; z < splits; z++) s[0] += ws[(size_t)z * n + i];
+  out[i] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 /* Adam on FP32 master parameters, refreshing their bf16 copies. */

@@ -54,8 +54,11 @@ def _blockwise_close(a, b, frac):
         o += n
 
 
-@pytest.mark.parametrize("name", ["double_box", "triple_ladder", "1L_2pt"])
+@pytest.mark.parametrize("name", ["double_box", "triple_ladder", "1L_2pt", "cubic5"])
 def test_network_forward_and_backward(engine, name):
+    """1L_2pt and double_box: one block per weight gradient; triple_ladder
+    (1024 rows) and cubic5 (16384 rows): split-K weight gradients and bias
+    column sums over row chunks."""
     prob, p = _setup(engine, name)
     lr = feyntrop_amd.Learner(engine, p, init_seed=7)
     rng = np.random.default_rng(2)
@@ -142,12 +145,11 @@ def _rel2(p, m):
 @pytest.mark.parametrize("name", ["triple_ladder", "double_box", "2L_3pt_tutorial", "cubic5"])
 def test_training_unbiased_and_lowers_variance(engine, name):
     """Training keeps the estimate unbiased (within 3 combined sigma of the
-    plain sampler's) and lowers the variance.  The training accumulates with
-    float atomics, so the network differs from run to run at rounding level,
-    and a single relative-variance estimate of 2e7 samples is heavy-tailed on
-    the double box (single-seed ratios from 0.3 to 3.4 were seen): the gain
-    is the ratio of the medians over three seeds of each sampler (measured
-    3.6-5.3 over eight trainings on the double box)."""
+    plain sampler's) and lowers the variance.  A single relative-variance
+    estimate of 2e7 samples is heavy-tailed on the double box (single-seed
+    ratios from 0.3 to 3.4 were seen): the gain is the ratio of the medians
+    over three seeds of each sampler (measured 3.6-5.3 over eight trainings
+    on the double box)."""
     prob, p = _setup(engine, name)
     lr = feyntrop_amd.Learner(engine, p, init_seed=1)
     st = lr.train(steps=150, samples=1 << 18, seed=100, lr=3e-3)
@@ -164,6 +166,30 @@ def test_training_unbiased_and_lowers_variance(engine, name):
     gain = np.median([_rel2(p, m) for m in mps]) / np.median([_rel2(p, m) for m in mls])
     print(f"[train] {name} gain {gain:.2f}", flush=True)
     assert gain > 1.5, gain
+
+
+@pytest.mark.parametrize("name", ["triple_ladder", "cubic5"])
+def test_training_is_reproducible(engine, name):
+    """The training statistics are summed in fixed point and the network's
+    split-K and column sums in a fixed order, so two trainings from the same
+    initial parameters and seeds give the same network bit for bit, and the
+    same learned estimate.  triple_ladder (E = 10) accumulates per block in
+    LDS, cubic5 (E = 14) the deeper states in global memory."""
+    prob, p = _setup(engine, name)
+    runs = []
+    for _ in range(2):
+        lr = feyntrop_amd.Learner(engine, p, init_seed=3)
+        st = lr.train(steps=12, samples=1 << 17, seed=40, lr=3e-3)
+        m = lr.integrate(1_000_000, 8)
+        runs.append((lr.params(), st[:, 0].copy(), m.arrays(), lr.tables()["lw"]))
+        lr.close()
+    (pa, sa, ma, la), (pb, sb, mb, lb) = runs
+    assert np.isfinite(pa).all() and np.abs(pa).max() > 0
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(sa, sb)
+    assert np.array_equal(la, lb)
+    for x, y in zip(ma, mb):
+        assert np.array_equal(x, y)
 
 
 def test_select_keeps_better_proposal(engine):
