@@ -333,6 +333,7 @@ def main():
     if not args.no_extras:
         extras = {}
         for key, fn in (("scan", lambda: run_scan(eng, args.scan_points)),
+                        ("wheel_k4", lambda: run_tuned(eng, "wheel_k4")),
                         ("cubic5", lambda: run_tuned(eng, "cubic5")),
                         ("triple_ladder", lambda: run_tuned(eng, "triple_ladder")),
                         ("large_graph", lambda: run_large(eng))):

@@ -521,8 +521,8 @@ int ftr_learner_train(ftr_ctx* c, ftr_learner* l, int steps, uint64_t samples, u
       {
         uint64_t n = samples;
         void* margs[] = {&L, &n};
-        HIPCHK(hipModuleLaunchKernel(l->f_hmx, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream,
-                                     margs, nullptr));
+        const unsigned mblocks = (unsigned)std::min<uint64_t>((samples + 1023) / 1024, c->prop.multiProcessorCount);
+        HIPCHK(hipModuleLaunchKernel(l->f_hmx, mblocks, 1, 1, 1024, 1, 1, 0, c->stream, margs, nullptr));
         void* hargs[] = {&L, &n, &l->d_part};
         if (E <= HIST_E)
           HIPCHK(hipModuleLaunchKernel(l->f_hst, (unsigned)l->hist_blocks, 1, 1, 1024, 1, 1,

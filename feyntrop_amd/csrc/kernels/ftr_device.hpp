@@ -1062,28 +1062,34 @@ __device__ inline double learn_fix_scale(const LearnLaunch& L, uint64_t N) {
 __device__ inline fixacc to_fix(double v, double scale) { return (fixacc)(long long)rint(v * scale); }
 
 /* *gmax = max |value| over the step's samples, as the bits of a
- * non-negative double (whose order is that of the integers). */
+ * non-negative double (whose order is that of the integers).  Grid-stride
+ * over the samples; one atomic per block (atomics on one address
+ * serialise). */
 template <int E>
 __device__ inline void learn_max_body(const LearnLaunch& L, uint64_t N) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double wm[16];
   double m = 0.0;
-  if (i < N) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
     const double c = L.gbuf[i];
-    if (c != 0.0 && finite_d(c)) {
-      m = fabs(c);
-      const uint64_t path = L.gpath[i];
-      uint32_t g = (1u << E) - 1u;
-      for (int j = 0; j < E - 1; j++) {
-        const uint32_t cg = g ^ (1u << (int)((path >> (4 * j)) & 15u));
-        const double sv = learn_radial(L, c, N, i, j, cg);
-        if (finite_d(sv)) m = fmax(m, fabs(sv));
-        g = cg;
-      }
+    if (c == 0.0 || !finite_d(c)) continue;
+    m = fmax(m, fabs(c));
+    const uint64_t path = L.gpath[i];
+    uint32_t g = (1u << E) - 1u;
+    for (int j = 0; j < E - 1; j++) {
+      const uint32_t cg = g ^ (1u << (int)((path >> (4 * j)) & 15u));
+      const double sv = learn_radial(L, c, N, i, j, cg);
+      if (finite_d(sv)) m = fmax(m, fabs(sv));
+      g = cg;
     }
   }
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0 && m > 0.0)
-    atomicMax((unsigned long long*)L.gmax, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wm[w]);
+    if (m > 0.0) atomicMax((unsigned long long*)L.gmax, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 /* E > FTR_LEARN_LDS_E: one thread per sample, adding into one accumulator
@@ -1300,7 +1306,7 @@ __device__ inline void batch_body(const BatchItem* items, int nitems) {
                                                                    double* w, uint64_t* path) {   \
     ftr::learn_sample_body<G, TMODE>(L, X, w, path);                                              \
   }                                                                                               \
-  extern "C" __global__ void __launch_bounds__(256) NAME##_hmx(ftr::LearnLaunch L, uint64_t N) {    \
+  extern "C" __global__ void __launch_bounds__(1024) NAME##_hmx(ftr::LearnLaunch L, uint64_t N) {   \
     ftr::learn_max_body<G::E>(L, N);                                                              \
   }                                                                                               \
   extern "C" __global__ void __launch_bounds__(256) NAME##_acc(ftr::LearnLaunch L, uint64_t N,      \
