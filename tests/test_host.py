@@ -43,6 +43,25 @@ def _host_prepare(prob):
 
 
 @pytest.mark.parametrize("name", list(W.all_problems()))
+def test_jtable_is_its_row_sums(oracle, name):
+    """The J recursion of tropical_sampling.hpp:258-276 and :411, bit for
+    bit: J(g) is the running sum, from 0.0 in ascending edge order, of
+    J(g\e)/r(g\e) over the edges of g, and so equals the last running sum
+    of the sampler's row for g (the absent edges' zeros leave the sum
+    unchanged).  Checked on the oracle's tables, which equal the
+    reference's."""
+    Jr, r, _, _ = oracle.problem(W.all_problems()[name]()).jtable()
+    E = int(round(np.log2(len(Jr))))
+    # (r of the full graph may be 0: it is never the state after a cut)
+    w = [float(Jr[g]) / float(r[g]) if r[g] != 0 else float("nan") for g in range(len(Jr))]
+    for g in range(1, len(Jr)):
+        T = 0.0
+        for e in range(E):
+            T += w[g ^ (1 << e)] if (g >> e) & 1 else 0.0
+        assert T == float(Jr[g]), (g, T, Jr[g])
+
+
+@pytest.mark.parametrize("name", list(W.all_problems()))
 def test_host_jtable_matches_oracle(oracle, name):
     prob = W.all_problems()[name]()
     hp = _host_prepare(prob)
