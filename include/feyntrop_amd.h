@@ -207,7 +207,9 @@ int ftr_learner_backward(ftr_ctx* ctx, ftr_learner* l, const float* dY, float* g
 /* `steps` Adam steps, each on `samples` proposal samples (stream seed
  * seed + step).  stats (optional, steps x 2): per step the relative
  * second moment N sum(w f)^2 / (sum w f)^2 of the epsilon^0 real part and
- * the kernel milliseconds. */
+ * the kernel milliseconds.  Reproducible: the same initial parameters,
+ * samples, seed and lr give the same network bit for bit (statistics in
+ * fixed point, gradient sums in a fixed order). */
 int ftr_learner_train(ftr_ctx* ctx, ftr_learner* l, int steps, uint64_t samples, uint64_t seed, double lr,
                       double* stats);
 /* Training objective: 0 forward KL between |f| p_trop and q (default; the
