@@ -116,6 +116,7 @@ def lib():
                                         C.POINTER(ftr_moments)]
     L.ftr_learner_sample.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp, vp]
     L.ftr_learner_set_objective.argtypes = [vp, C.c_int]
+    L.ftr_learner_statistics.argtypes = [vp, vp, C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp, vp, vp]
     L.ftr_learner_select.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
@@ -412,6 +413,19 @@ class Learner:
         self._chk(lib().ftr_learner_train(self.engine.h, self.h, int(steps), int(samples), int(seed),
                                           float(lr), st.ctypes.data))
         return st[:steps]
+
+    def statistics(self, samples, seed):
+        """One training step's record and forward-KL statistics, without the
+        update: dict of c (N), ly ((E-1) x N), path (N), gC (2^E x E), gS
+        (2^E), gsum (2) and dY (rows x 32)."""
+        n, E, N = 1 << self.E, self.E, int(samples)
+        out = dict(c=np.zeros(N), ly=np.zeros((max(E - 1, 1), N)), path=np.zeros(N, np.uint64),
+                   gC=np.zeros((n, E)), gS=np.zeros(n), gsum=np.zeros(2), dY=np.zeros((self.rows, 32), np.float32))
+        self._chk(lib().ftr_learner_statistics(self.engine.h, self.h, N, int(seed), out["c"].ctypes.data,
+                                               out["ly"].ctypes.data, out["path"].ctypes.data,
+                                               out["gC"].ctypes.data, out["gS"].ctypes.data,
+                                               out["gsum"].ctypes.data, out["dY"].ctypes.data))
+        return out
 
     def tables(self):
         n, E = 1 << self.E, self.E

@@ -294,6 +294,67 @@ int fail_exc(ftr_ctx* c, std::exception& e) {
 
 }  // namespace
 
+namespace {
+
+/* the training record (c, radial log-draws, cut order) for `samples` */
+void ensure_records(ftr_learner* l, uint64_t samples) {
+  if (samples <= l->rec_cap) return;
+  if (l->d_gbuf) HIPCHK(hipFree(l->d_gbuf));
+  if (l->d_gpath) HIPCHK(hipFree(l->d_gpath));
+  l->d_gbuf = nullptr;
+  l->d_gpath = nullptr;
+  l->rec_cap = 0;
+  HIPCHK(hipMalloc((void**)&l->d_gbuf, (size_t)samples * l->E * 8));
+  HIPCHK(hipMalloc((void**)&l->d_gpath, (size_t)samples * 8));
+  l->rec_cap = samples;
+}
+
+/* One training step up to dL/dY (d_dY): network forward and tables, the
+ * proposal samples with their record, the fixed-point forward-KL statistics
+ * (gC, gS, gsum) and the table gradient.  ev0/ev1 bracket the sampling and
+ * the statistics. */
+Geometry step_statistics(ftr_ctx* c, ftr_learner* l, uint64_t samples, uint64_t seed) {
+  ftr_problem* p = l->p;
+  const int E = l->E, NQ = 2 + 4 * p->prep.K;
+  const size_t acc_lds = (size_t)(2 + 2 * E + E * E + E * (E - 1) / 2 * (1 + E)) * 8;
+  forward(c, l);
+  build_tables(c, l);
+  /* fixed-point statistics: the scale's maximum and (E > HIST_E) the one
+   * accumulator start at zero (zero bits) */
+  fill_d(c, l, reinterpret_cast<double*>(l->d_gmax), 1, 0.0);
+  if (E > HIST_E) fill_d(c, l, reinterpret_cast<double*>(l->d_part), hist_len(E), 0.0);
+  Geometry G = setup_stream(p, 0, samples, seed, 0, l->var.block, NQ, l->f_lgr);
+  ftr::LearnLaunch L = learn_launch(l, G, samples);
+  void* args[] = {&L};
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  HIPCHK(hipModuleLaunchKernel(l->f_lgr, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
+                               c->stream, args, nullptr));
+  uint64_t n = samples;
+  void* margs[] = {&L, &n};
+  const unsigned mblocks = (unsigned)std::min<uint64_t>((samples + 1023) / 1024, c->prop.multiProcessorCount);
+  HIPCHK(hipModuleLaunchKernel(l->f_hmx, mblocks, 1, 1, 1024, 1, 1, 0, c->stream, margs, nullptr));
+  void* hargs[] = {&L, &n, &l->d_part};
+  if (E <= HIST_E)
+    HIPCHK(hipModuleLaunchKernel(l->f_hst, (unsigned)l->hist_blocks, 1, 1, 1024, 1, 1,
+                                 (unsigned)(hist_len(E) * 8), c->stream, hargs, nullptr));
+  else
+    HIPCHK(hipModuleLaunchKernel(l->f_acc, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1, (unsigned)acc_lds,
+                                 c->stream, hargs, nullptr));
+  int nb = l->hist_blocks;
+  void* rargs[] = {&L, &l->d_part, &nb, &n};
+  HIPCHK(hipModuleLaunchKernel(l->f_hrd, (unsigned)((hist_len(E) + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream,
+                               rargs, nullptr));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  fill_f(c, l, l->d_dY, (uint64_t)l->M * NOUT, 0.0f);
+  int e = E;
+  double dmax = DMAX, smax = SMAX;
+  void* targs[] = {&l->d_Y, &e, &l->d_lw, &l->d_lZ, &l->d_gC, &l->d_gS, &l->d_gsum, &l->d_dY, &dmax, &smax};
+  launch(l->mlp.tgrad, (l->NT + 255) / 256, 1, 256, c->stream, targs);
+  return G;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learner** out) {
@@ -491,58 +552,11 @@ int ftr_learner_train(ftr_ctx* c, ftr_learner* l, int steps, uint64_t samples, u
   ftr_problem* p = l->p;
   try {
     HIPCHK(hipSetDevice(c->device));
-    const int K = p->prep.K, NQ = 2 + 4 * K;
+    const int K = p->prep.K;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    if (samples > l->rec_cap) {
-      if (l->d_gbuf) HIPCHK(hipFree(l->d_gbuf));
-      if (l->d_gpath) HIPCHK(hipFree(l->d_gpath));
-      l->d_gbuf = nullptr;
-      l->d_gpath = nullptr;
-      l->rec_cap = 0;
-      HIPCHK(hipMalloc((void**)&l->d_gbuf, (size_t)samples * l->E * 8));
-      HIPCHK(hipMalloc((void**)&l->d_gpath, (size_t)samples * 8));
-      l->rec_cap = samples;
-    }
-    const int E = l->E;
-    const size_t acc_lds = (size_t)(2 + 2 * E + E * E + E * (E - 1) / 2 * (1 + E)) * 8;
+    ensure_records(l, samples);
     for (int st = 0; st < steps; st++) {
-      forward(c, l);
-      build_tables(c, l);
-      /* fixed-point statistics: the scale's maximum and (E > HIST_E) the
-       * one accumulator start at zero (zero bits) */
-      fill_d(c, l, reinterpret_cast<double*>(l->d_gmax), 1, 0.0);
-      if (E > HIST_E) fill_d(c, l, reinterpret_cast<double*>(l->d_part), hist_len(E), 0.0);
-      Geometry G = setup_stream(p, 0, samples, seed + (uint64_t)st, 0, l->var.block, NQ, l->f_lgr);
-      ftr::LearnLaunch L = learn_launch(l, G, samples);
-      void* args[] = {&L};
-      HIPCHK(hipEventRecord(c->ev0, c->stream));
-      HIPCHK(hipModuleLaunchKernel(l->f_lgr, (unsigned)G.nblocks, 1, 1, l->var.block, 1, 1, (unsigned)l->lds,
-                                   c->stream, args, nullptr));
-      {
-        uint64_t n = samples;
-        void* margs[] = {&L, &n};
-        const unsigned mblocks = (unsigned)std::min<uint64_t>((samples + 1023) / 1024, c->prop.multiProcessorCount);
-        HIPCHK(hipModuleLaunchKernel(l->f_hmx, mblocks, 1, 1, 1024, 1, 1, 0, c->stream, margs, nullptr));
-        void* hargs[] = {&L, &n, &l->d_part};
-        if (E <= HIST_E)
-          HIPCHK(hipModuleLaunchKernel(l->f_hst, (unsigned)l->hist_blocks, 1, 1, 1024, 1, 1,
-                                       (unsigned)(hist_len(E) * 8), c->stream, hargs, nullptr));
-        else
-          HIPCHK(hipModuleLaunchKernel(l->f_acc, (unsigned)((samples + 255) / 256), 1, 1, 256, 1, 1,
-                                       (unsigned)acc_lds, c->stream, hargs, nullptr));
-        int nb = l->hist_blocks;
-        void* rargs[] = {&L, &l->d_part, &nb, &n};
-        HIPCHK(hipModuleLaunchKernel(l->f_hrd, (unsigned)((hist_len(E) + 255) / 256), 1, 1, 256, 1, 1, 0,
-                                     c->stream, rargs, nullptr));
-      }
-      HIPCHK(hipEventRecord(c->ev1, c->stream));
-      fill_f(c, l, l->d_dY, (uint64_t)l->M * NOUT, 0.0f);
-      {
-        int E = l->E;
-        double dmax = DMAX, smax = SMAX;
-        void* targs[] = {&l->d_Y, &E, &l->d_lw, &l->d_lZ, &l->d_gC, &l->d_gS, &l->d_gsum, &l->d_dY, &dmax, &smax};
-        launch(l->mlp.tgrad, (l->NT + 255) / 256, 1, 256, c->stream, targs);
-      }
+      Geometry G = step_statistics(c, l, samples, seed + (uint64_t)st);
       backward(c, l);
       l->adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)l->adam_t), c2 = 1.0f - std::pow(b2, (float)l->adam_t);
@@ -566,6 +580,28 @@ int ftr_learner_train(ftr_ctx* c, ftr_learner* l, int steps, uint64_t samples, u
     }
     l->tables_valid = false;
     ensure_tables(c, l);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (std::exception& e) {
+    return fail_exc(c, e);
+  }
+  return FTR_OK;
+}
+
+int ftr_learner_statistics(ftr_ctx* c, ftr_learner* l, uint64_t samples, uint64_t seed, double* cw, double* ly,
+                           uint64_t* path, double* gC, double* gS, double* gsum, float* dY) {
+  if (!c || !l || samples == 0) return FTR_ERR_INVALID;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    ensure_records(l, samples);
+    step_statistics(c, l, samples, seed);
+    const size_t NT = l->NT, E = l->E, N = samples;
+    if (cw) HIPCHK(hipMemcpyDtoHAsync(cw, l->d_gbuf, N * 8, c->stream));
+    if (ly && E > 1) HIPCHK(hipMemcpyDtoHAsync(ly, l->d_gbuf + N, (E - 1) * N * 8, c->stream));
+    if (path) HIPCHK(hipMemcpyDtoHAsync(path, l->d_gpath, N * 8, c->stream));
+    if (gC) HIPCHK(hipMemcpyDtoHAsync(gC, l->d_gC, NT * E * 8, c->stream));
+    if (gS) HIPCHK(hipMemcpyDtoHAsync(gS, l->d_gS, NT * 8, c->stream));
+    if (gsum) HIPCHK(hipMemcpyDtoHAsync(gsum, l->d_gsum, 2 * 8, c->stream));
+    if (dY) HIPCHK(hipMemcpyDtoHAsync(dY, l->d_dY, (size_t)l->M * NOUT * 4, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   } catch (std::exception& e) {
     return fail_exc(c, e);

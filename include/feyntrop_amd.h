@@ -212,6 +212,19 @@ int ftr_learner_backward(ftr_ctx* ctx, ftr_learner* l, const float* dY, float* g
  * fixed point, gradient sums in a fixed order). */
 int ftr_learner_train(ftr_ctx* ctx, ftr_learner* l, int steps, uint64_t samples, uint64_t seed, double lr,
                       double* stats);
+/* One training step's statistics and table gradient, without the update
+ * (for checking them): draws `samples` proposal samples (stream seed `seed`)
+ * and returns, each output optional (NULL skips it):
+ *   cw[N]         c = |f| p_trop/q per sample (its square with objective 1;
+ *                 0 for a dropped sample)
+ *   ly[(E-1) N]   the radial log-draws, ly[j N + i] for step j of sample i
+ *   path[N]       the cut order, 4 bits per step
+ *   gC[2^E E], gS[2^E], gsum[2]   the forward-KL statistics (ftr_abi.hpp
+ *                 LearnLaunch), summed in fixed point
+ *   dY[max(32, 2^E) x 32]         dL/dY of the network outputs
+ * No reference counterpart (learned proposal). */
+int ftr_learner_statistics(ftr_ctx* ctx, ftr_learner* l, uint64_t samples, uint64_t seed, double* cw, double* ly,
+                           uint64_t* path, double* gC, double* gS, double* gsum, float* dY);
 /* Training objective: 0 forward KL between |f| p_trop and q (default; the
  * statistics weight each path by c = |f| p_trop/q), 1 the estimator's
  * variance E_q[(f p_trop/q)^2] (c = (|f| p_trop/q)^2). */

@@ -192,6 +192,66 @@ def test_training_is_reproducible(engine, name):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("name,objective", [("triple_ladder", 0), ("cubic5", 0), ("double_box", 1)])
+def test_training_statistics_and_gradient(engine, name, objective):
+    """One step's forward-KL statistics against numpy on the step's own
+    record (c, radial log-draws, cut order), and dL/dY against the formula
+    on those statistics.  Bars: C, S and sum c within the fixed-point
+    rounding, 2^-k / 2 per added value (k from the step's largest value as
+    the kernels fix it), plus 1e-12 relative for numpy's summation order;
+    the sample count exact; dY to FP32 rounding.  triple_ladder and
+    double_box sum per block in LDS, cubic5 (E = 14) the deeper states in
+    global memory."""
+    prob, p = _setup(engine, name)
+    lr = feyntrop_amd.Learner(engine, p, init_seed=5)
+    lr.set_params(_rand_par(np.random.default_rng(4)))
+    if objective:
+        lr.set_objective(objective)
+    N = 1 << 16
+    st = lr.statistics(N, 77)
+    t = lr.tables()
+    E, n = p.E, 1 << p.E
+    c, ly, path = st["c"], st["ly"], st["path"].astype(np.uint64)
+    assert st["gsum"][1] == N
+    ok = (c != 0) & np.isfinite(c)
+    C = np.zeros((n, E)); S = np.zeros(n); nC = np.zeros((n, E)); nS = np.zeros(n)
+    g = np.full(N, n - 1, np.int64)
+    M = np.abs(c[ok]).max()
+    for j in range(E):
+        e = ((path >> np.uint64(4 * j)) & np.uint64(15)).astype(np.int64)
+        np.add.at(C, (g[ok], e[ok]), c[ok])
+        np.add.at(nC, (g[ok], e[ok]), 1.0)
+        cg = g ^ (1 << e)
+        if j < E - 1:
+            sv = c * (1.0 + ly[j] / t["lia"][cg])
+            good = ok & np.isfinite(sv)
+            np.add.at(S, cg[good], sv[good])
+            np.add.at(nS, cg[good], 1.0)
+            M = max(M, np.abs(sv[good]).max())
+        g = cg
+    k = min(max(62 - np.frexp(M)[1] - np.frexp(float(N))[1], -1000), 1000)
+    ulp = 2.0 ** -k
+    assert np.all(np.abs(st["gC"] - C) <= 0.5 * ulp * nC + 1e-12 * np.abs(C))
+    assert np.all(np.abs(st["gS"] - S) <= 0.5 * ulp * nS + 1e-12 * np.abs(S))
+    assert abs(st["gsum"][0] - c[ok].sum()) <= 0.5 * ulp * ok.sum() + 1e-12 * c[ok].sum()
+    # dL/dY from the statistics (mlp_table_grad)
+    Y = lr.forward().astype(np.float64)[:n]
+    gC, gS = st["gC"], st["gS"]
+    Ssum = st["gsum"][0] if st["gsum"][0] > 0 else 1.0
+    visits = gC.sum(axis=1)
+    present = ((np.arange(n)[:, None] >> np.arange(E)[None, :]) & 1).astype(bool)
+    q = np.where(present, t["lw"] / np.where(t["lZ"] > 0, t["lZ"], 1.0)[:, None], 0.0)
+    th = np.tanh(Y[:, :E] / 8.0)
+    ref = np.zeros((n, 32))
+    ref[:, :E] = np.where(present, -(gC - q * visits[:, None]) / Ssum * (1.0 - th * th), 0.0)
+    ts = np.tanh(Y[:, E] / 0.6)
+    ref[:, E] = -gS / Ssum * (1.0 - ts * ts)
+    dY = st["dY"][:n].astype(np.float64)
+    scale = np.abs(ref).max()
+    assert scale > 0
+    assert np.all(np.abs(dY - ref) <= 1e-6 * np.abs(ref) + 1e-6 * scale)
+
+
 def test_select_keeps_better_proposal(engine):
     """Plain sampling judges the network: a trained learner on the triple
     ladder is kept; a learner with a deliberately bad tilt (random output
