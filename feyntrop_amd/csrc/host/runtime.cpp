@@ -166,6 +166,9 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
   }
   s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
   s << "  static constexpr bool DDL = " << (dd_in_lds(v, g.E) ? "true" : "false") << ";\n";
+  /* in-step row prefetch of the sampler (ftr_device.hpp sample_point):
+   * generic deformed-path kernels */
+  s << "  static constexpr bool PF = " << ((v.mode == 2 && !v.cparams) ? "true" : "false") << ";\n";
   s << (v.cparams ? "  using PRM = FP;\n" : "  using PRM = ftr::DevParams;\n");
   arr(s, "hi", g.hi);
   arr(s, "lo", g.lo);
