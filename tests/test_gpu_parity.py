@@ -66,6 +66,31 @@ def gpu_replay(engine, p, X, px):
     return out.cpu().numpy().reshape(n, p.K, 2), drop.cpu().numpy()
 
 
+@pytest.mark.parametrize("name", ["wheel_k4", "double_box", "cubic5"])
+def test_tiny_runs_are_the_stream_prefix(engine, name):
+    """N from 1 to a few thousand: fewer samples than lanes, a partial last
+    block, lanes with fewer than their share.  The moments are those of the
+    stream's first N samples replayed one by one (sample kernel + replay
+    kernel; 1e-11 relative: the replay takes psi and xi instead of their
+    logs), and every sample is counted or dropped."""
+    print(f"[tiny] {name}", flush=True)
+    p = engine.prepare(W.all_problems()[name]())
+    X, px = gpu_sample(engine, p, 0, 5000, seed=9)
+    vals, drop = gpu_replay(engine, p, X, px)
+    parts = (0, 1) if p.info.path == 2 else (0,)
+    for N in (1, 2, 3, 5, 63, 65, 257, 4999):
+        m = engine.integrate(p, N, 9)
+        n, s1, s2 = m.arrays()
+        ok = drop[:N] == 0
+        assert m.dropped == int((~ok).sum()), N
+        for k in range(p.K):
+            for q in parts:
+                v = vals[:N][ok, k, q]
+                assert n[k, q] == ok.sum(), (N, k, q)
+                assert abs(s1[k, q] - v.sum()) <= 1e-11 * max(np.abs(v).sum(), 1e-300), (N, k, q)
+                assert abs(s2[k, q] - (v * v).sum()) <= 1e-11 * max((v * v).sum(), 1e-300), (N, k, q)
+
+
 @pytest.mark.parametrize("name", SMALL)
 def test_sampler_matches_reference_stream(engine, golden_problems, name):
     """Samples 0..39 of seed 0 against the compiled reference (golden), and
