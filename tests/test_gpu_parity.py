@@ -417,6 +417,34 @@ def test_batch_equals_individual(engine):
         assert np.allclose(s2b, s2i, rtol=1e-10, atol=0)
 
 
+def test_batch_ragged_and_empty(engine):
+    """Ragged batches: N = 0, 1, 2, 5, ... next to 3e5 in one launch, each
+    equal to its own call; a problem with N = 0 gets zero sums; an empty
+    batch is a no-op."""
+    import feyntrop_amd
+    assert engine.integrate_batch([], [], []) == []
+    pts = W.scan_points(12, seed=11, accept=feyntrop_amd.host_accepts)
+    probs = [engine.prepare(p) for p in pts]
+    Ns = [0, 1, 2, 5, 17, 300_000, 63, 65, 1, 4096, 3, 299_999][:len(probs)]
+    seeds = [p["seed"] for p in pts]
+    outs = engine.integrate_batch(probs, Ns, seeds)
+    for q, N, sd, mb in zip(probs, Ns, seeds, outs):
+        nb, s1b, s2b = mb.arrays()
+        if N == 0:
+            # no samples: zero sums; the real path's imaginary parts carry the
+            # reference's two zero updates (feynman_integral.hpp:130-139)
+            assert not s1b.any() and not s2b.any() and mb.dropped == 0
+            assert not nb[:, 0].any()
+            assert np.all(nb[:, 1] == (0 if q.info.path == 2 else 2))
+            continue
+        mi = engine.integrate(q, N, sd)
+        ni, s1i, s2i = mi.arrays()
+        assert np.array_equal(nb, ni) and mb.dropped == mi.dropped
+        assert nb[0][0] + mb.dropped == N
+        assert np.allclose(s1b, s1i, rtol=1e-10, atol=1e-12 * max(np.abs(s1i).max(), 1e-300))
+        assert np.allclose(s2b, s2i, rtol=1e-10, atol=0)
+
+
 def test_autotune_keeps_results(engine, tmp_path):
     """Every kernel variant the autotuner may pick draws the same samples."""
     import feyntrop_amd
