@@ -374,25 +374,41 @@ static Module module_for(ftr_ctx* c, const ftrh::Prep& pr, const Variant& v) {
   return m;
 }
 
-/* Jump polynomials of a launch geometry: lane jumps S*per*2^j (j < lb),
- * then, for base-16 digits k < ndig of the block index, the jumps of
- * d 16^k blocks (d = 1..15); cached per context. */
+/* Lane jump polynomials in base 16 from the jump of one lane (q1): entry
+ * 15 k + d - 1 of `out` jumps d 16^k lanes (d = 1..15, k < (lb+3)/4).
+ * Returns the jump of 2^lb lanes (one block). */
+static ftrh::Poly lane_digit_polys(const ftrh::Poly& q1, int lb, uint64_t* out) {
+  const int ldig = (lb + 3) / 4;
+  ftrh::Poly q = q1; /* 16^k lanes */
+  for (int k = 0; k < ldig; k++) {
+    ftrh::Poly r = q;
+    for (int d = 1; d <= 15; d++) {
+      memcpy(out + 4 * (15 * (size_t)k + d - 1), r.w, 32);
+      if (d < 15) r = ftrh::xo_poly_mul(r, q);
+    }
+    for (int sq = 0; sq < 4; sq++) q = ftrh::xo_poly_square(q);
+  }
+  ftrh::Poly blk = q1;
+  for (int j = 0; j < lb; j++) blk = ftrh::xo_poly_square(blk);
+  return blk;
+}
+
+/* Jump polynomials of a launch geometry: the lane digit jumps
+ * (lane_digit_polys), then, for base-16 digits k < ndig of the block index,
+ * the jumps of d 16^k blocks (d = 1..15); cached per context. */
 static const std::vector<uint64_t>& jump_polys(ftr_ctx* c, uint64_t S, uint64_t per, int lb, int ndig) {
   const auto key = std::make_tuple(S, per, lb, ndig);
   auto it = c->jump_cache.find(key);
   if (it != c->jump_cache.end()) return it->second;
-  std::vector<uint64_t> polys(4 * ((size_t)lb + 15 * (size_t)ndig));
+  const size_t nlane = 15 * (size_t)((lb + 3) / 4);
+  std::vector<uint64_t> polys(4 * (nlane + 15 * (size_t)ndig));
   const unsigned __int128 dl = (unsigned __int128)S * per;
-  ftrh::Poly q = ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64));
-  for (int j = 0; j < lb; j++) {
-    memcpy(&polys[4 * j], q.w, 32);
-    q = ftrh::xo_poly_square(q);
-  }
-  /* q is now the jump polynomial for one block (S * per * block) */
+  /* q: the jump polynomial for one block (S * per * block) */
+  ftrh::Poly q = lane_digit_polys(ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64)), lb, polys.data());
   for (int k = 0; k < ndig; k++) {
     ftrh::Poly r = q;
     for (int d = 1; d <= 15; d++) {
-      memcpy(&polys[4 * ((size_t)lb + 15 * k + d - 1)], r.w, 32);
+      memcpy(&polys[4 * (nlane + 15 * k + d - 1)], r.w, 32);
       if (d < 15) r = ftrh::xo_poly_mul(r, q);
     }
     for (int sq = 0; sq < 4; sq++) q = ftrh::xo_poly_square(q); /* 16^(k+1) blocks */
@@ -449,7 +465,7 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
     HIPCHK(hipModuleGetFunction(&c->f_block_states, mod, "ftrj_block_states"));
   }
   {
-    const uint64_t* bp = G.d_polys + 4 * lb;
+    const uint64_t* bp = G.d_polys + 4 * 15 * (size_t)((lb + 3) / 4);
     int nb = ndig;
     uint64_t nbl = nblocks;
     void* args[] = {&d_base, &bp, &nb, &nbl, &G.d_states};
@@ -996,7 +1012,7 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
       int lb = 0;
       while ((1 << lb) < g.block) lb++;
       g.poly_off = n_polys;
-      n_polys += (size_t)std::max(lb, 1);
+      n_polys += std::max<size_t>(15 * (size_t)((lb + 3) / 4), 1);
       g.nblocks = 0;
       for (int i : g.idx) {
         const uint64_t lanes = (Ns[i] + g.S - 1) / g.S;
@@ -1033,14 +1049,11 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
       const int E = probs[g.idx[0]]->prep.g.E;
       const uint64_t per = 2 * (uint64_t)E - 1;
       const unsigned __int128 dl = (unsigned __int128)g.S * per;
-      ftrh::Poly q = ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64));
       int lb = 0;
       while ((1 << lb) < g.block) lb++;
-      for (int j = 0; j < lb; j++) {
-        memcpy(&polys[4 * (g.poly_off + j)], q.w, 32);
-        q = ftrh::xo_poly_square(q);
-      }
-      const ftrh::Poly qb = q; /* one block: S * per * block */
+      /* one block: S * per * block */
+      const ftrh::Poly qb =
+          lane_digit_polys(ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64)), lb, &polys[4 * g.poly_off]);
       uint32_t b0 = 0;
       for (size_t t = 0; t < g.idx.size(); t++) {
         const int i = g.idx[t];

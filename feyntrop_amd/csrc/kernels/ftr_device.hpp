@@ -831,16 +831,35 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-/* Lane stream positioning: the block's first state comes from the host;
- * lane t applies the jump polynomials of the set bits of t. */
+/* state <- q(M) state for a jump polynomial q that differs between lanes
+ * (the accumulation is masked per lane instead of branched on). */
+__device__ inline void apply_poly_lane(Xoshiro& g, const uint64_t* q) {
+  Xoshiro acc{0, 0, 0, 0};
+  for (int w = 0; w < 4; w++) {
+    const uint64_t bits = q[w];
+    for (int b = 0; b < 64; b++) {
+      const uint64_t m = 0ull - ((bits >> b) & 1ull);
+      acc.s0 ^= g.s0 & m;
+      acc.s1 ^= g.s1 & m;
+      acc.s2 ^= g.s2 & m;
+      acc.s3 ^= g.s3 & m;
+      g.next();
+    }
+  }
+  g = acc;
+}
+
+/* Lane stream positioning: the block's first state comes from the device
+ * (ftrj_block_states); lane t applies, for each base-16 digit d_k of t, the
+ * jump polynomial of d_k 16^k lanes (lane_poly[15 k + d_k - 1]): two
+ * polynomials for 256-thread blocks. */
 __device__ inline Xoshiro lane_stream(const Launch& a, uint32_t bid) {
   const uint64_t* bs = a.block_state + 4 * (size_t)bid;
   Xoshiro g{bs[0], bs[1], bs[2], bs[3]};
   const unsigned t = threadIdx.x;
-  for (int j = 0; (1u << j) < blockDim.x; j++) {
-    Xoshiro h = g;
-    apply_poly(h, a.lane_poly + 4 * j);
-    if ((t >> j) & 1) g = h;
+  for (int k = 0; (1u << (4 * k)) < blockDim.x; k++) {
+    const unsigned d = (t >> (4 * k)) & 15u;
+    if (d) apply_poly_lane(g, a.lane_poly + 4 * (15 * k + d - 1));
   }
   return g;
 }
