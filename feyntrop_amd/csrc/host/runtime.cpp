@@ -394,24 +394,24 @@ static ftrh::Poly lane_digit_polys(const ftrh::Poly& q1, int lb, uint64_t* out) 
 }
 
 /* Jump polynomials of a launch geometry: the lane digit jumps
- * (lane_digit_polys), then, for base-16 digits k < ndig of the block index,
- * the jumps of d 16^k blocks (d = 1..15); cached per context. */
+ * (lane_digit_polys), then, for base-256 digits k < ndig of the block index,
+ * the jumps of d 256^k blocks (d = 1..255); cached per context. */
 static const std::vector<uint64_t>& jump_polys(ftr_ctx* c, uint64_t S, uint64_t per, int lb, int ndig) {
   const auto key = std::make_tuple(S, per, lb, ndig);
   auto it = c->jump_cache.find(key);
   if (it != c->jump_cache.end()) return it->second;
   const size_t nlane = 15 * (size_t)((lb + 3) / 4);
-  std::vector<uint64_t> polys(4 * (nlane + 15 * (size_t)ndig));
+  std::vector<uint64_t> polys(4 * (nlane + 255 * (size_t)ndig));
   const unsigned __int128 dl = (unsigned __int128)S * per;
   /* q: the jump polynomial for one block (S * per * block) */
   ftrh::Poly q = lane_digit_polys(ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64)), lb, polys.data());
   for (int k = 0; k < ndig; k++) {
     ftrh::Poly r = q;
-    for (int d = 1; d <= 15; d++) {
-      memcpy(&polys[4 * (nlane + 15 * k + d - 1)], r.w, 32);
-      if (d < 15) r = ftrh::xo_poly_mul(r, q);
+    for (int d = 1; d <= 255; d++) {
+      memcpy(&polys[4 * (nlane + 255 * (size_t)k + d - 1)], r.w, 32);
+      if (d < 255) r = ftrh::xo_poly_mul(r, q);
     }
-    for (int sq = 0; sq < 4; sq++) q = ftrh::xo_poly_square(q); /* 16^(k+1) blocks */
+    for (int sq = 0; sq < 8; sq++) q = ftrh::xo_poly_square(q); /* 256^(k+1) blocks */
   }
   return c->jump_cache.emplace(key, std::move(polys)).first->second;
 }
@@ -440,7 +440,7 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
   int lb = 0;
   while ((1 << lb) < block) lb++;
   int ndig = 0;
-  while (ndig < 16 && ((uint64_t)1 << (4 * ndig)) < nblocks) ndig++;
+  while (ndig < 8 && ((uint64_t)1 << (8 * ndig)) < nblocks) ndig++;
   const std::vector<uint64_t>& polys = jump_polys(c, S, per, lb, ndig);
   /* the stream state at sample `first`; block b starts b blocks later
    * (ftrj_block_states on the device) */

@@ -256,9 +256,10 @@ __device__ inline void jt_level_body(const JtParams* __restrict__ prm, int n, ui
 }  // namespace ftrj
 
 /* Stream start states of the blocks of a launch: state of block b = the
- * base state jumped by b blocks.  b is written in base 16; digit k with
- * value d > 0 applies the jump polynomial of d 16^k blocks, polys[(15 k +
- * d - 1) 4 ...] (a jump by polynomial q maps the state s to q(M) s, M the
+ * base state jumped by b blocks.  b is written in base 256 (one digit for up
+ * to 256 blocks); digit k with value d > 0 applies the jump polynomial of
+ * d 256^k blocks, polys[(255 k + d - 1) 4 ...] (a jump by polynomial q maps
+ * the state s to q(M) s, M the
  * xoshiro256+ transition over GF(2); jumps commute, so any decomposition
  * gives the same state).  One thread per block. */
 __device__ inline void xo_apply_dev(uint64_t (&s)[4], const uint64_t* q) {
@@ -293,8 +294,8 @@ extern "C" __global__ void __launch_bounds__(256) ftrj_block_states(const uint64
   if (b >= nblocks) return;
   uint64_t s[4] = {base[0], base[1], base[2], base[3]};
   for (int k = 0; k < ndig; k++) {
-    const int d = (int)((b >> (4 * k)) & 15);
-    if (d) xo_apply_dev(s, polys + 4 * (15 * k + d - 1));
+    const int d = (int)((b >> (8 * k)) & 255);
+    if (d) xo_apply_dev(s, polys + 4 * (255 * k + d - 1));
   }
   out[4 * b + 0] = s[0];
   out[4 * b + 1] = s[1];
