@@ -374,19 +374,22 @@ static Module module_for(ftr_ctx* c, const ftrh::Prep& pr, const Variant& v) {
   return m;
 }
 
-/* Lane jump polynomials in base 16 from the jump of one lane (q1): entry
- * 15 k + d - 1 of `out` jumps d 16^k lanes (d = 1..15, k < (lb+3)/4).
+/* Lane jump polynomials in base 256 from the jump of one lane (q1): entry
+ * 255 k + d - 1 of `out` jumps d 256^k lanes (d = 1..255, k < (lb+7)/8;
+ * the top digit of a block's lanes uses the first entries only).
  * Returns the jump of 2^lb lanes (one block). */
+static size_t lane_poly_count(int lb) { return 255 * (size_t)((lb + 7) / 8); }
 static ftrh::Poly lane_digit_polys(const ftrh::Poly& q1, int lb, uint64_t* out) {
-  const int ldig = (lb + 3) / 4;
-  ftrh::Poly q = q1; /* 16^k lanes */
+  const int ldig = (lb + 7) / 8;
+  ftrh::Poly q = q1; /* 256^k lanes */
   for (int k = 0; k < ldig; k++) {
+    const int top = (k == ldig - 1) ? (1 << (lb - 8 * k)) - 1 : 255; /* largest digit used */
     ftrh::Poly r = q;
-    for (int d = 1; d <= 15; d++) {
-      memcpy(out + 4 * (15 * (size_t)k + d - 1), r.w, 32);
-      if (d < 15) r = ftrh::xo_poly_mul(r, q);
+    for (int d = 1; d <= top; d++) {
+      memcpy(out + 4 * (255 * (size_t)k + d - 1), r.w, 32);
+      if (d < top) r = ftrh::xo_poly_mul(r, q);
     }
-    for (int sq = 0; sq < 4; sq++) q = ftrh::xo_poly_square(q);
+    for (int sq = 0; sq < 8; sq++) q = ftrh::xo_poly_square(q);
   }
   ftrh::Poly blk = q1;
   for (int j = 0; j < lb; j++) blk = ftrh::xo_poly_square(blk);
@@ -400,7 +403,7 @@ static const std::vector<uint64_t>& jump_polys(ftr_ctx* c, uint64_t S, uint64_t 
   const auto key = std::make_tuple(S, per, lb, ndig);
   auto it = c->jump_cache.find(key);
   if (it != c->jump_cache.end()) return it->second;
-  const size_t nlane = 15 * (size_t)((lb + 3) / 4);
+  const size_t nlane = lane_poly_count(lb);
   std::vector<uint64_t> polys(4 * (nlane + 255 * (size_t)ndig));
   const unsigned __int128 dl = (unsigned __int128)S * per;
   /* q: the jump polynomial for one block (S * per * block) */
@@ -465,7 +468,7 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
     HIPCHK(hipModuleGetFunction(&c->f_block_states, mod, "ftrj_block_states"));
   }
   {
-    const uint64_t* bp = G.d_polys + 4 * 15 * (size_t)((lb + 3) / 4);
+    const uint64_t* bp = G.d_polys + 4 * lane_poly_count(lb);
     int nb = ndig;
     uint64_t nbl = nblocks;
     void* args[] = {&d_base, &bp, &nb, &nbl, &G.d_states};
@@ -1012,7 +1015,7 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
       int lb = 0;
       while ((1 << lb) < g.block) lb++;
       g.poly_off = n_polys;
-      n_polys += std::max<size_t>(15 * (size_t)((lb + 3) / 4), 1);
+      n_polys += std::max<size_t>(lane_poly_count(lb), 1);
       g.nblocks = 0;
       for (int i : g.idx) {
         const uint64_t lanes = (Ns[i] + g.S - 1) / g.S;

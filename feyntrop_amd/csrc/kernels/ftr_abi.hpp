@@ -60,8 +60,8 @@ struct Launch {
   const double* t_rows;    /* table mode 3: 2^E x E, row g holds w(g\e) for e in g, 0 else */
   const StateRec* t_rec;   /* table mode 3: 2^E records */
   const uint64_t* block_state; /* nblocks x 4: stream state at the block's first sample */
-  const uint64_t* lane_poly;   /* 15 per base-16 digit of the lane index, x 4 words: entry
-                                * 15 k + d - 1 jumps d 16^k lanes (S (2E-1) draws each) */
+  const uint64_t* lane_poly;   /* 255 per base-256 digit of the lane index, x 4 words: entry
+                                * 255 k + d - 1 jumps d 256^k lanes (S (2E-1) draws each) */
   uint64_t N;              /* total samples */
   uint64_t S;              /* samples per lane */
   double* partials;        /* nblocks x (2 + 4K) */

@@ -850,16 +850,16 @@ __device__ inline void apply_poly_lane(Xoshiro& g, const uint64_t* q) {
 }
 
 /* Lane stream positioning: the block's first state comes from the device
- * (ftrj_block_states); lane t applies, for each base-16 digit d_k of t, the
- * jump polynomial of d_k 16^k lanes (lane_poly[15 k + d_k - 1]): two
- * polynomials for 256-thread blocks. */
+ * (ftrj_block_states); lane t applies, for each base-256 digit d_k of t, the
+ * jump polynomial of d_k 256^k lanes (lane_poly[255 k + d_k - 1]): one
+ * polynomial for blocks of up to 256 threads. */
 __device__ inline Xoshiro lane_stream(const Launch& a, uint32_t bid) {
   const uint64_t* bs = a.block_state + 4 * (size_t)bid;
   Xoshiro g{bs[0], bs[1], bs[2], bs[3]};
   const unsigned t = threadIdx.x;
-  for (int k = 0; (1u << (4 * k)) < blockDim.x; k++) {
-    const unsigned d = (t >> (4 * k)) & 15u;
-    if (d) apply_poly_lane(g, a.lane_poly + 4 * (15 * k + d - 1));
+  for (int k = 0; (1u << (8 * k)) < blockDim.x; k++) {
+    const unsigned d = (t >> (8 * k)) & 255u;
+    if (d) apply_poly_lane(g, a.lane_poly + 4 * (255 * k + d - 1));
   }
   return g;
 }
