@@ -232,12 +232,41 @@ LARGE = {
 def all_problems():
     d = dict(CONFIGS)
     d.update(FIXTURES)
+    d.update(DROPS)
     return d
 
 
-# Batched scan: 1- to 3-loop topologies with 4-10 edges and 3-7 vertices,
+def drop_double_box(N=10**5, seed=0, D=5.5, K=2):
+    """The double box's Minkowski kinematics with lambda = 0: no deformation,
+    so the real path runs on indefinite kinematics (the reference does the
+    same).  The second Symanzik pxi changes sign over the domain: with a
+    non-integer W (D = 5.5: W = 1.5) pxi^-W is NaN where pxi < 0 and those
+    samples are dropped and counted (feynman_integral.hpp:254-261); with an
+    integer W (D = 6) the eps^0 values stay finite but log(pxi) is NaN, so
+    the eps^k (k >= 1) sums are NaN and the output prints them as null
+    (feyntrop.cpp:264-278)."""
+    p = double_box(N, seed)
+    p["lambda"] = 0
+    p["dimension"] = D
+    p["num_eps_terms"] = K
+    return p
+
+
+DROPS = {
+    "drop_double_box_D5.5": lambda N=10**5, seed=0: drop_double_box(N, seed, 5.5, 2),
+    "drop_double_box_D6": lambda N=10**5, seed=0: drop_double_box(N, seed, 6.0, 2),
+}
+
+
+# Batched scan: 1- to 3-loop topologies with 3-10 edges and 3-8 vertices,
 # (edges, external vertices).
 SCAN_TOPOLOGIES = {
+    "triangle": ([(0, 1), (1, 2), (2, 0)], (0, 1, 2)),
+    # 3-loop Mercedes: the K4 wheel (hub 0, rim 1-2-3) with a vertex on
+    # each rim edge, 4 external legs
+    "mercedes": ([(0, 1), (0, 2), (0, 3), (1, 4), (4, 2), (2, 5), (5, 3), (3, 6), (6, 1)], (1, 4, 5, 6)),
+    "triple_ladder": ([(0, 1), (2, 3), (4, 5), (6, 7), (0, 2), (2, 4), (4, 6), (1, 3), (3, 5), (5, 7)],
+                      (0, 1, 6, 7)),
     "box": ([(0, 1), (1, 2), (2, 3), (3, 0)], (0, 1, 2, 3)),
     "pentagon": ([(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], (0, 1, 2, 3, 4)),
     "kite": ([(0, 2), (0, 3), (1, 2), (1, 3), (2, 3)], (0, 1)),
@@ -308,7 +337,7 @@ def scan_problems(path_of=None):
     their kernels can be specialised ahead of time.  path_of(prob) returns the
     path (0, 1, 2) or None when the preprocessing rejects the point."""
     reps = {}
-    for prob in scan_points(1024, seed=12345):
+    for prob in scan_points(3072, seed=12345):
         path = path_of(prob) if path_of else (2 if prob["lambda"] else 0)
         if path is None:
             continue

@@ -34,6 +34,12 @@ def golden_rng():
 
 
 @pytest.fixture(scope="session")
+def replay_stats():
+    with open(os.path.join(GOLDEN, "replay_stats.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="session")
 def oracle():
     from oracle_lib import Oracle
     return Oracle()

@@ -17,6 +17,17 @@ Outputs (all data, no reference source):
   fixtures.json   the reference test suite's expected results
                   (tests/*.txt), the README output of the K4 wheel with
                   masses [1..6] and the tutorial notebook's printed result
+  cli.json        outputs of the reference binary (oracle/_ref/feyntrop_ref)
+                  run with one thread: the stdout line and the number of
+                  dropped-sample warnings for the problems that drop samples
+                  or print null (workloads.DROPS), and the diagnostic lines
+                  written before integration (or the error text) for the
+                  graphs with 13 or more edges of tests/diag_cases.py
+  replay_stats.json  per problem, how far the reference's own build (its
+                  -ffast-math flags) is from the strict oracle on the 100k
+                  sample points of stream seed 11: fraction of samples above
+                  1e-12 relative, median, p99 and maximum (eps^0), and
+                  whether both drop the same samples
 """
 
 import ast
@@ -104,7 +115,71 @@ def main():
                                 [(-435.06, 1.30), (-174.30, 1.33)], [(-191.81, 2.15), (-494.58, 2.14)],
                                 [(219.08, 2.69), (-431.85, 2.67)]]
     json.dump(fx, open(os.path.join(HERE, "fixtures.json"), "w"), indent=1)
+    cli_fixtures()
+    replay_stats()
+
+
+REPLAY_STATS_PROBLEMS = ["wheel_k4", "readme_wheel", "double_box", "cubic5", "triple_ladder"] + \
+    list(W.FIXTURES) + list(W.DROPS)
+
+
+def replay_stats(n=100000):
+    from oracle_lib import Oracle
+    O, R = Oracle(), Ref()
+    out = {}
+    for name in REPLAY_STATS_PROBLEMS:
+        prob = W.all_problems()[name]()
+        op, rp = O.problem(prob), R.problem(prob)
+        X, px, _ = op.sample(O.seed(11), n)
+        o, od, _ = op.replay(X, px)
+        r, rd, _ = rp.values(O.seed(11), n)
+        keep = (od == 0) & (rd == 0)
+        zo = o[keep, 0, 0] + 1j * o[keep, 0, 1]
+        zr = r[keep, 0, 0] + 1j * r[keep, 0, 1]
+        d = np.abs(zr - zo) / np.maximum(np.abs(zo), 1e-300)
+        out[name] = {"n": n, "frac_gt_1e-12": float(np.mean(d > 1e-12)), "median": float(np.median(d)),
+                     "p99": float(np.quantile(d, 0.99)), "max": float(d.max()),
+                     "same_drops": bool(np.array_equal(od, rd))}
+        print(name, out[name])
+    json.dump(out, open(os.path.join(HERE, "replay_stats.json"), "w"), indent=1)
+
+
+def _ref_cli(prob, N):
+    import subprocess
+    from oracle_lib import REF_BIN
+    q = {k: v for k, v in prob.items() if not k.startswith("_")}
+    q["N"], q["seed"] = int(N), 0
+    return subprocess.run([REF_BIN], input=json.dumps(q), capture_output=True, text=True, timeout=900,
+                          env={"OMP_NUM_THREADS": "1", "PATH": os.environ.get("PATH", "")})
+
+
+def cli_fixtures():
+    from diag_cases import large_cases
+    out = {"drops": {}, "diagnostics_large": {}}
+    for name, fn in W.DROPS.items():
+        prob = fn()
+        r = _ref_cli(prob, prob["N"])
+        assert r.returncode == 0, r.stderr[-2000:]
+        out["drops"][name] = {"N": prob["N"], "stdout": r.stdout.strip(),
+                              "warnings": r.stderr.count("Warning: Sampled value number")}
+        print(name, r.stdout.strip())
+    for name, prob in large_cases().items():
+        r = _ref_cli(prob, 1000)
+        lines = r.stderr.splitlines()
+        if r.returncode != 0:
+            out["diagnostics_large"][name] = {
+                "error": [l.split("what():", 1)[1].strip() for l in lines if "what():" in l]}
+        else:
+            start = [i for i, l in enumerate(lines) if l.startswith("Started integrating")][0]
+            out["diagnostics_large"][name] = {"lines": lines[:start]}
+        print(name, out["diagnostics_large"][name])
+    json.dump(out, open(os.path.join(HERE, "cli.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["cli"]:
+        cli_fixtures()
+    elif sys.argv[1:] == ["replay_stats"]:
+        replay_stats()
+    else:
+        main()

@@ -1,0 +1,151 @@
+"""The drop rule and the null output on the GPU (reference:
+feynman_integral.hpp:227-234, :254-261, :329-347 drop a non-finite R_0, count
+nothing for it and warn; feyntrop.cpp:264-278 prints NaN sums as null).
+
+Trigger: the double box's Minkowski kinematics with lambda = 0, which the
+reference integrates on the real path although the second Symanzik pxi
+changes sign (workloads.DROPS):
+  * D = 5.5 (W = 1.5): pxi^-W is NaN wherever pxi < 0, so 83 % of the
+    samples are dropped;
+  * D = 6 (W = 1): R_0 stays finite, log(pxi) is NaN, so the eps^1 sums are
+    NaN and print as null.
+
+Bars: the counts and drops of the reference's one-thread run exactly; the
+means of the kept samples within 1e-6 of their standard error; replayed
+drop masks identical to the oracle's; the CLI's null entries where the
+reference binary prints them, its numbers equal to the reference's to the
+same bar, and its warning line naming the reference's number of dropped
+samples."""
+
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from feyntrop_amd import workloads as W
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+NAMES = list(W.DROPS)
+
+
+def _hex(a):
+    return np.array([float.fromhex(x) for x in a])
+
+
+def _crel(a, b):
+    za = a[..., 0] + 1j * a[..., 1]
+    zb = b[..., 0] + 1j * b[..., 1]
+    return np.abs(za - zb) / np.maximum(np.abs(zb), 1e-300)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_drops_equal_reference_one_thread(engine, golden_problems, name):
+    """integrate(N, seed 0) keeps and drops exactly the reference's samples."""
+    print(f"[drops-estimate] {name}", flush=True)
+    prob = W.DROPS[name]()
+    p = engine.prepare(prob)
+    assert p.info.path in (0, 1) and p.info.lambda_eff == 0.0
+    g = golden_problems[name]["estimate"]
+    N = g["N"]
+    ref = _hex(g["moments"]).reshape(p.K, 2, 3)
+    m = engine.integrate(p, N, 0)
+    n, s1, s2 = m.arrays()
+    print(f"   N {N} kept {int(n[0, 0])} dropped {m.dropped}", flush=True)
+    assert m.dropped == N - int(ref[0, 0, 0])
+    if name.endswith("D5.5"):
+        assert m.dropped > N // 2
+    for k in range(p.K):
+        assert n[k, 0] == ref[k, 0, 0]
+        # the real path's imaginary parts: the reference's two zero updates
+        assert n[k, 1] == 2 and s1[k, 1] == 0.0 and s2[k, 1] == 0.0
+        rn, rs1, rs2 = ref[k, 0]
+        if not np.isfinite(rs1):
+            assert not np.isfinite(s1[k, 0]), (k, s1[k, 0])
+            continue
+        sig = np.sqrt(max(rs2 / rn - (rs1 / rn) ** 2, 0) / rn)
+        assert abs(s1[k, 0] - rs1) / rn <= 1e-6 * sig, (k, s1[k, 0], rs1)
+        assert abs(s2[k, 0] - rs2) <= 1e-8 * abs(rs2), (k, s2[k, 0], rs2)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_drop_mask_on_replay(engine, oracle, oracle_ld, replay_stats, name):
+    """100k oracle points: the GPU drops exactly the samples the strict oracle
+    drops.  On the kept ones (pxi crosses zero in the domain, so some sit
+    next to a pole) the values agree with it as the reference's own build
+    does: median 1e-14, the fraction above 1e-12 at most 2x the reference
+    build's plus 1e-4 (tests/golden/replay_stats.json), and 1e-12 except a
+    0.5 % tail where the strict oracle is within 1e-13 of long double."""
+    print(f"[drops-replay] {name}", flush=True)
+    prob = W.DROPS[name]()
+    p = engine.prepare(prob)
+    op = oracle.problem(prob)
+    ol = oracle_ld.problem(prob)
+    n = 100000
+    X, px, _ = op.sample(oracle.seed(11), n)
+    ref, rdrop, _ = op.replay(X, px)
+    tru, _, _ = ol.replay(X, px)
+    Xd = torch.from_numpy(np.ascontiguousarray(X.reshape(-1))).cuda()
+    pd = torch.from_numpy(np.ascontiguousarray(px.reshape(-1))).cuda()
+    out = torch.zeros(n * p.K * 2, dtype=torch.float64, device="cuda")
+    drop = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    engine.replay(p, n, Xd.data_ptr(), pd.data_ptr(), out.data_ptr(), drop.data_ptr())
+    out = out.cpu().numpy().reshape(n, p.K, 2)
+    drop = drop.cpu().numpy().astype(bool)
+    rdrop = rdrop.astype(bool)
+    print(f"   dropped {drop.sum()} of {n}", flush=True)
+    assert rdrop.any() == name.endswith("D5.5")
+    assert np.array_equal(drop, rdrop)
+    keep = ~rdrop
+    d = _crel(out[keep, 0:1], ref[keep, 0:1])[:, 0]
+    well = _crel(ref[keep, 0:1], tru[keep, 0:1])[:, 0] <= 1e-13
+    rs = replay_stats[name]
+    print(f"   kept {keep.sum()} median {np.median(d):.2e} well-conditioned {well.mean():.4f} "
+          f"frac>1e-12 {np.mean(d > 1e-12):.5f} (reference build {rs['frac_gt_1e-12']:.5f}; "
+          f"on the mask {np.mean(d[well] > 1e-12):.5f}) max {d.max():.2e}", flush=True)
+    assert np.median(d) <= 1e-14
+    assert np.mean(d > 1e-12) <= 2.0 * rs["frac_gt_1e-12"] + 1e-4
+    assert np.mean(d[well] > 1e-12) <= 0.005
+
+
+def _cli(prob):
+    import feyntrop_amd
+    q = {k: v for k, v in prob.items() if not k.startswith("_")}
+    return subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps(q), capture_output=True, text=True, timeout=300)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_cli_null_and_warning_equal_reference(name):
+    """The binary on the drop problems: null exactly where the reference
+    binary (one thread, the same samples) prints null, the other numbers
+    equal to its own up to summation order, and the warning line names its
+    number of dropped samples."""
+    import os
+    print(f"[drops-cli] {name}", flush=True)
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cli.json")) as f:
+        gold = json.load(f)["drops"][name]
+    prob = W.DROPS[name](N=gold["N"])
+    out = _cli(prob)
+    assert out.returncode == 0, out.stderr
+    line = out.stdout.strip()
+    ours, ref = json.loads(line), json.loads(gold["stdout"])
+    assert list(ours) == list(ref)
+    assert ours["IGtr"] == ref["IGtr"]
+    for k, (a, b) in enumerate(zip(ours["integral"], ref["integral"])):
+        for q in range(2):
+            (va, ea), (vb, eb) = a[q], b[q]
+            assert (va is None) == (vb is None) and (ea is None) == (eb is None), (k, q, a, b)
+            if vb is None:
+                continue
+            assert abs(va - vb) <= 1e-6 * eb + 1e-300, (k, q, va, vb, eb)
+            assert abs(ea - eb) <= 1e-6 * eb + 1e-300, (k, q, ea, eb)
+    if gold["warnings"]:
+        assert f"Warning: {gold['warnings']} sampled values were not finite" in out.stderr, out.stderr
+    else:
+        assert "were not finite" not in out.stderr
+    if any(v is None for row in ref["integral"] for part in row for v in part):
+        assert "null" in line

@@ -37,6 +37,17 @@ def _hex(a):
     return np.array([float.fromhex(x) for x in a])
 
 
+def _record(kind, name, rec):
+    """Per-problem parity statistics: printed by the tests, and written as
+    JSON under $FEYNTROP_AMD_PARITY_OUT when that directory is given."""
+    out = os.environ.get("FEYNTROP_AMD_PARITY_OUT")
+    if not out:
+        return
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, f"{kind}__{name}.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
 def _dev(n):
     return torch.zeros(int(n), dtype=torch.float64, device="cuda")
 
@@ -125,7 +136,11 @@ def test_sampler_jump_ahead(engine, oracle, name):
 @pytest.mark.parametrize("name", SMALL)
 def test_replay_on_reference_points(engine, golden_problems, name):
     """GPU integrand on the reference's own first 40 sample points vs the
-    reference's per-sample values (from its own N=1 estimates)."""
+    reference's per-sample values (from its own N=1 estimates).  Real paths:
+    every sample within 1e-12 (the strict oracle is within 2e-13 of the
+    reference on these points).  Deformed path: median 1e-13, at most 5 %
+    above 1e-12 and none above 1e-9 (the strict oracle itself reaches
+    1.2e-12 on one triple-ladder point)."""
     print(f"[replay-golden] {name}", flush=True)
     p = engine.prepare(W.all_problems()[name]())
     g = golden_problems[name]
@@ -135,35 +150,68 @@ def test_replay_on_reference_points(engine, golden_problems, name):
     out, drop = gpu_replay(engine, p, X, px)
     assert not drop.any()
     r = _crel(out, ref)
+    print(f"   path {p.info.path} median {np.median(r):.2e} max {r.max():.2e} frac>1e-12 {np.mean(r > 1e-12):.3f}",
+          flush=True)
+    _record("replay_reference_points", name, {"path": p.info.path, "n": int(r.size), "median": float(np.median(r)),
+                                               "max": float(r.max()), "frac_gt_1e-12": float(np.mean(r > 1e-12))})
     assert np.median(r) <= 1e-13
-    assert np.mean(r > 1e-12) <= 0.05
+    if p.info.path != 2:
+        assert r.max() <= 1e-12
+    else:
+        assert np.mean(r > 1e-12) <= 0.05
+        assert r.max() <= 1e-9
 
 
 @pytest.mark.parametrize("name", SMALL)
-def test_replay_parity_mask(engine, oracle, oracle_ld, name):
-    """100k oracle points: accuracy against the long-double oracle matches
-    the reference algorithm's own, and on the well-conditioned mask the GPU
-    equals the strict oracle to 1e-12 (allowing a 0.5% tail, max 1e-8)."""
+def test_replay_parity_mask(engine, oracle, oracle_fma, oracle_ld, replay_stats, name):
+    """100k oracle points.  Accuracy against the long-double oracle matches
+    the reference algorithm's own (quantiles).  Against the strict oracle,
+    over all samples: the fraction above 1e-12 at most 2x that of the
+    reference's own build (its -ffast-math flags, same points;
+    tests/golden/replay_stats.json) plus 1e-4, the 99th percentile at most
+    10x its own plus 1e-14.  On the well-conditioned mask (strict within
+    1e-13 of long double): 1e-12 except a 0.5 % tail, none above 1e-8.  The
+    statistics are recorded per problem."""
     print(f"[replay-mask] {name}", flush=True)
     prob = W.all_problems()[name]()
     p = engine.prepare(prob)
-    op, ol = oracle.problem(prob), oracle_ld.problem(prob)
+    op, ol, of = oracle.problem(prob), oracle_ld.problem(prob), oracle_fma.problem(prob)
     n = 100000
     X, px, _ = op.sample(oracle.seed(11), n)
     ref, rdrop, _ = op.replay(X, px)
     tru, _, _ = ol.replay(X, px)
+    alt, _, _ = of.replay(X, px)
     out, drop = gpu_replay(engine, p, X, px)
     assert np.array_equal(drop.astype(bool), rdrop.astype(bool))
     e_gpu = _crel(out, tru)[:, 0]
     e_ref = _crel(ref, tru)[:, 0]
     for q in (0.5, 0.9, 0.99, 0.999):
         assert np.quantile(e_gpu, q) <= 2.0 * np.quantile(e_ref, q) + 2e-15, q
+    d_all = _crel(out, ref)[:, 0]
+    d_cpu = _crel(alt, ref)[:, 0]
     mask = e_ref <= 1e-13
-    d = _crel(out, ref)[:, 0][mask]
-    print(f"   mask {mask.mean():.4f} median {np.median(d):.2e} frac>1e-12 {np.mean(d > 1e-12):.5f} max {d.max():.2e}")
+    d = d_all[mask]
+    rec = {"path": p.info.path, "n": n, "mask_frac": float(mask.mean()),
+           "all": {"median": float(np.median(d_all)), "p99": float(np.quantile(d_all, 0.99)),
+                   "frac_gt_1e-12": float(np.mean(d_all > 1e-12)), "max": float(d_all.max())},
+           "mask": {"median": float(np.median(d)), "p99": float(np.quantile(d, 0.99)),
+                    "frac_gt_1e-12": float(np.mean(d > 1e-12)), "max": float(d.max())},
+           "cpu_fma_vs_strict": {"frac_gt_1e-12": float(np.mean(d_cpu > 1e-12)), "max": float(d_cpu.max())},
+           "reference_build_vs_strict": replay_stats[name],
+           "vs_long_double": {q: float(np.quantile(e_gpu, float(q))) for q in ("0.5", "0.99", "0.999")},
+           "reference_algorithm_vs_long_double": {q: float(np.quantile(e_ref, float(q))) for q in ("0.5", "0.99", "0.999")}}
+    print(f"   path {p.info.path} mask {mask.mean():.4f} | all: median {rec['all']['median']:.2e} p99 "
+          f"{rec['all']['p99']:.2e} frac>1e-12 {rec['all']['frac_gt_1e-12']:.5f} max {rec['all']['max']:.2e} "
+          f"(reference build {replay_stats[name]['frac_gt_1e-12']:.5f}, max {replay_stats[name]['max']:.1e}) "
+          f"| mask: median {np.median(d):.2e} "
+          f"frac>1e-12 {np.mean(d > 1e-12):.5f} max {d.max():.2e}", flush=True)
+    _record("replay_mask", name, rec)
     assert np.median(d) <= 1e-14
     assert np.mean(d > 1e-12) <= 0.005
     assert d.max() <= 1e-8
+    rs = replay_stats[name]
+    assert rec["all"]["frac_gt_1e-12"] <= 2.0 * rs["frac_gt_1e-12"] + 1e-4
+    assert rec["all"]["p99"] <= 10.0 * rs["p99"] + 1e-14
 
 
 @pytest.mark.parametrize("name", SMALL)
