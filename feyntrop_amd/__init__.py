@@ -84,6 +84,8 @@ def lib():
     L.ftr_problem_info.argtypes = [vp, C.POINTER(ftr_info)]
     L.ftr_jtable.argtypes = [vp, vp, vp, vp, vp]
     L.ftr_integrate.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts), C.POINTER(ftr_moments)]
+    L.ftr_integrate_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts),
+                                        C.POINTER(ftr_moments)]
     L.ftr_finalize.argtypes = [vp, C.POINTER(ftr_moments), vp]
     L.ftr_sample.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp]
     L.ftr_replay.argtypes = [vp, vp, C.c_uint64, vp, vp, vp, vp]
@@ -283,6 +285,16 @@ class Engine:
         m = ftr_moments()
         opts = ftr_opts(0, -1, int(lanes))
         rc = lib().ftr_integrate(self.h, p.h, int(N), int(seed), C.byref(opts), C.byref(m))
+        if rc:
+            raise self._err(rc)
+        return m
+
+    def integrate_noalloc(self, p, N, seed=0, lanes=0):
+        """ftr_integrate_noalloc: the same integration with no device
+        allocation (scratch in a fixed per-GPU device array)."""
+        m = ftr_moments()
+        opts = ftr_opts(0, -1, int(lanes))
+        rc = lib().ftr_integrate_noalloc(self.h, p.h, int(N), int(seed), C.byref(opts), C.byref(m))
         if rc:
             raise self._err(rc)
         return m

@@ -19,6 +19,11 @@ extern const char* ftr_embedded_device_source;
 extern const char* ftr_embedded_mlp_source;
 extern const char* ftr_embedded_jtable_source;
 
+/* noalloc.hip */
+extern "C" hipError_t ftrn_scratch_init(int device, void** ptr);
+extern "C" hipError_t ftrn_sum_partials_launch(const double* partials, uint64_t nblocks, int nq, double* tot,
+                                               hipStream_t s);
+
 using ftrh::Error;
 
 struct Variant {
@@ -45,6 +50,9 @@ struct ftr_ctx {
   std::map<std::tuple<uint64_t, uint64_t, int, int>, std::vector<uint64_t>> jump_cache;
   hipFunction_t f_block_states = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  /* this device's fixed scratch of ftr_integrate_noalloc (noalloc.hip) */
+  void* na_scratch = nullptr;
+  std::string na_err;
   /* growable device scratch: block states, lane polys, partials */
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -96,6 +104,7 @@ struct Geometry {
   uint64_t* d_states = nullptr;
   uint64_t* d_polys = nullptr;
   double* d_partials = nullptr;
+  double* d_totals = nullptr;   /* nq doubles after the partials */
 };
 
 /* kernel source of a problem's variant; `learned` adds the learned-proposal
@@ -107,8 +116,13 @@ ftr::DevParams make_params(const ftrh::Prep& pr);
 std::string code_object(const std::string& src, std::string* path_out);
 /* load (once per context) and resolve the kernels named `names` */
 hipModule_t load_module_raw(ftr_ctx* c, const std::string& src);
+/* ext (optional): a caller-provided device scratch of ext_bytes instead of
+ * the context's growable one; a launch that does not fit is an FTR_ERR_LIMIT
+ * error. */
 Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
-                      int nq, hipFunction_t occ_fn);
+                      int nq, hipFunction_t occ_fn, char* ext = nullptr, size_t ext_bytes = 0);
+/* the kernel that writes the block start states (loaded once per context) */
+hipFunction_t block_states_fn(ftr_ctx* c);
 void ensure_scratch(ftr_ctx* c, size_t bytes);
 void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
 Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit = (size_t)128 << 30);

@@ -642,6 +642,9 @@ int ftr_learner_integrate(ftr_ctx* c, ftr_learner* l, uint64_t N, uint64_t seed,
     if (l->use_plain) return ftr_integrate(c, p, N, seed, opts, out);
     if (opts && opts->block && opts->block != l->var.block)
       throw Error(FTR_ERR_INVALID, "block size differs from the compiled variant");
+    if (opts && opts->table_mode != -1 && opts->table_mode != l->var.tmode)
+      throw Error(FTR_ERR_INVALID, "table_mode " + std::to_string(opts->table_mode) +
+                                       " differs from the compiled placement " + std::to_string(l->var.tmode));
     ensure_tables(c, l);
     if (N > 0) {
       Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, l->var.block, NQ, l->f_lev);

@@ -419,8 +419,16 @@ static const std::vector<uint64_t>& jump_polys(ftr_ctx* c, uint64_t S, uint64_t 
   return c->jump_cache.emplace(key, std::move(polys)).first->second;
 }
 
+hipFunction_t block_states_fn(ftr_ctx* c) {
+  if (!c->f_block_states) {
+    hipModule_t mod = load_module_raw(c, ftr_embedded_jtable_source);
+    HIPCHK(hipModuleGetFunction(&c->f_block_states, mod, "ftrj_block_states"));
+  }
+  return c->f_block_states;
+}
+
 Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
-                      int nq, hipFunction_t occ_fn) {
+                      int nq, hipFunction_t occ_fn, char* ext, size_t ext_bytes) {
   ftr_ctx* c = p->ctx;
   const int E = p->prep.g.E;
   Geometry G;
@@ -444,36 +452,47 @@ Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed,
   while ((1 << lb) < block) lb++;
   int ndig = 0;
   while (ndig < 8 && ((uint64_t)1 << (8 * ndig)) < nblocks) ndig++;
+  const size_t bytes_states = (size_t)nblocks * 32;
+  const size_t bytes_polys = 32 * (lane_poly_count(lb) + 255 * (size_t)ndig);
+  const size_t bytes_part = (size_t)nblocks * nq * 8;
+  const size_t need = ((bytes_states + 255) & ~(size_t)255) + ((bytes_polys + 255) & ~(size_t)255) + bytes_part +
+                      (size_t)nq * 8 + 512;
+  char* d;
+  if (ext) {
+    if (need > ext_bytes)
+      throw Error(FTR_ERR_LIMIT, "launch of " + std::to_string(nblocks) + " blocks needs " + std::to_string(need) +
+                                     " bytes of scratch, more than the fixed " + std::to_string(ext_bytes) +
+                                     " bytes of the no-allocation entry point (request fewer lanes)");
+    d = ext;
+  } else {
+    ensure_scratch(c, need);
+    d = (char*)c->d_scratch;
+  }
   const std::vector<uint64_t>& polys = jump_polys(c, S, per, lb, ndig);
   /* the stream state at sample `first`; block b starts b blocks later
-   * (ftrj_block_states on the device) */
-  std::vector<uint64_t> base(4);
-  ftrh::xo_seed(seed, base.data());
+   * (ftrj_block_states on the device, which takes it by value) */
+  uint64_t base[4];
+  ftrh::xo_seed(seed, base);
   {
     const unsigned __int128 d0 = (unsigned __int128)first * per;
-    if (d0) ftrh::xo_apply(ftrh::xo_jump_poly((uint64_t)d0, (uint64_t)(d0 >> 64)), base.data());
+    if (d0) ftrh::xo_apply(ftrh::xo_jump_poly((uint64_t)d0, (uint64_t)(d0 >> 64)), base);
   }
-  const size_t bytes_states = (size_t)nblocks * 32, bytes_polys = polys.size() * 8 + 32;
-  const size_t bytes_part = (size_t)nblocks * nq * 8;
-  ensure_scratch(c, bytes_states + bytes_polys + bytes_part + 512);
-  char* d = (char*)c->d_scratch;
   G.d_states = (uint64_t*)d;
   G.d_polys = (uint64_t*)(d + ((bytes_states + 255) & ~(size_t)255));
   G.d_partials = (double*)((char*)G.d_polys + ((bytes_polys + 255) & ~(size_t)255));
-  uint64_t* d_base = G.d_polys + polys.size();
+  G.d_totals = G.d_partials + (size_t)nblocks * nq;
+  /* the polynomials live in the context's cache, so the asynchronous copy
+   * reads memory that outlives it */
   if (!polys.empty()) HIPCHK(hipMemcpyHtoDAsync(G.d_polys, (void*)polys.data(), polys.size() * 8, c->stream));
-  HIPCHK(hipMemcpyHtoDAsync(d_base, base.data(), 32, c->stream));
-  if (!c->f_block_states) {
-    hipModule_t mod = load_module_raw(c, ftr_embedded_jtable_source);
-    HIPCHK(hipModuleGetFunction(&c->f_block_states, mod, "ftrj_block_states"));
-  }
   {
+    hipFunction_t f = block_states_fn(c);
     const uint64_t* bp = G.d_polys + 4 * lane_poly_count(lb);
     int nb = ndig;
     uint64_t nbl = nblocks;
-    void* args[] = {&d_base, &bp, &nb, &nbl, &G.d_states};
-    HIPCHK(hipModuleLaunchKernel(c->f_block_states, (unsigned)((nblocks + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream,
-                                 args, nullptr));
+    uint64_t b0 = base[0], b1 = base[1], b2 = base[2], b3 = base[3];
+    void* args[] = {&b0, &b1, &b2, &b3, &bp, &nb, &nbl, &G.d_states};
+    HIPCHK(hipModuleLaunchKernel(f, (unsigned)((nblocks + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, args,
+                                 nullptr));
   }
   return G;
 }
@@ -516,6 +535,18 @@ int ftr_create(int device, ftr_ctx** out) {
     if (const char* m = getenv("FEYNTROP_AMD_JTABLE")) {
       if (!strcmp(m, "host")) c->jtable_mode = 1;
       else if (!strcmp(m, "device")) c->jtable_mode = 2;
+    }
+    /* the block start-state kernel, loaded here so that no integration
+     * call has to load a module (ftr_integrate_noalloc allocates nothing) */
+    block_states_fn(c);
+    /* the fixed scratch of ftr_integrate_noalloc, resolved once per device
+     * with the device current; its failure only disables that entry point */
+    {
+      const hipError_t e = ftrn_scratch_init(device, &c->na_scratch);
+      if (e != hipSuccess) {
+        c->na_scratch = nullptr;
+        c->na_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+      }
     }
   } catch (Error& e) {
     fprintf(stderr, "feyntrop_amd: %s\n", e.what());
@@ -839,13 +870,19 @@ int ftr_jtable(const ftr_problem* p, double* Jr, double* r, uint8_t* mm, uint8_t
   return FTR_OK;
 }
 
-int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts, ftr_moments* out) {
+/* ftr_integrate and ftr_integrate_noalloc: one launch of the problem's
+ * integration kernel over N samples.  noalloc: all scratch in the device's
+ * fixed array, the block partials summed on the device in block order (the
+ * host loop's additions, in its order) and only the totals copied back. */
+static int integrate_impl(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts,
+                          ftr_moments* out, bool noalloc) {
   if (!c) {
     g_host_err = "no HIP device context: the problem was prepared host-only";
     return FTR_ERR_DEVICE;
   }
   if (!p || !out) return FTR_ERR_INVALID;
   if (!p->m.f_int) return fail(c, FTR_ERR_INVALID, "problem was prepared without a device context");
+  if (p->ctx != c) return fail(c, FTR_ERR_INVALID, "problem belongs to another context");
   try {
     const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(c->device));
@@ -854,8 +891,15 @@ int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const f
     out->K = K;
     if (opts && opts->block && opts->block != p->var.block)
       throw Error(FTR_ERR_INVALID, "block size differs from the compiled variant");
+    if (opts && opts->table_mode != -1 && opts->table_mode != p->var.tmode)
+      throw Error(FTR_ERR_INVALID, "table_mode " + std::to_string(opts->table_mode) +
+                                       " differs from the compiled placement " + std::to_string(p->var.tmode) +
+                                       " (choose it with ftr_set_variant)");
+    if (noalloc && !c->na_scratch)
+      throw Error(FTR_ERR_DEVICE, "the no-allocation scratch of this device is unavailable: " + c->na_err);
     if (N > 0) {
-      Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ, nullptr);
+      Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ, nullptr,
+                                noalloc ? (char*)c->na_scratch : nullptr, noalloc ? FTR_NOALLOC_SCRATCH_BYTES : 0);
       ftr::Launch a;
       fill_tables(a, p);
       a.block_state = G.d_states;
@@ -868,17 +912,24 @@ int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const f
       HIPCHK(hipModuleLaunchKernel(p->m.f_int, (unsigned)G.nblocks, 1, 1, p->var.block, 1, 1, (unsigned)p->lds,
                                    c->stream, args, nullptr));
       HIPCHK(hipEventRecord(c->ev1, c->stream));
-      c->h_partials.resize((size_t)G.nblocks * NQ);
-      HIPCHK(hipMemcpyDtoHAsync(c->h_partials.data(), G.d_partials, c->h_partials.size() * 8, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+      double tot[2 + 4 * FTR_MAX_K];
+      if (noalloc) {
+        HIPCHK(ftrn_sum_partials_launch(G.d_partials, G.nblocks, NQ, G.d_totals, c->stream));
+        HIPCHK(hipMemcpyDtoHAsync(tot, G.d_totals, (size_t)NQ * 8, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      } else {
+        c->h_partials.resize((size_t)G.nblocks * NQ);
+        HIPCHK(hipMemcpyDtoHAsync(c->h_partials.data(), G.d_partials, c->h_partials.size() * 8, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < NQ; q++) tot[q] = 0.0;
+        for (uint64_t b = 0; b < G.nblocks; b++)
+          for (int q = 0; q < NQ; q++) tot[q] += c->h_partials[b * NQ + q];
+      }
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       out->kernel_ms = ms;
       out->lanes = G.lanes;
       out->samples_per_lane = G.S;
-      std::vector<double> tot(NQ, 0.0);
-      for (uint64_t b = 0; b < G.nblocks; b++)
-        for (int q = 0; q < NQ; q++) tot[q] += c->h_partials[b * NQ + q];
       out->dropped = (uint64_t)tot[1];
       for (int k = 0; k < K; k++) {
         out->n[k][0] = tot[0];
@@ -902,6 +953,15 @@ int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const f
     return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
   }
   return FTR_OK;
+}
+
+int ftr_integrate(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts, ftr_moments* out) {
+  return integrate_impl(c, p, N, seed, opts, out, false);
+}
+
+int ftr_integrate_noalloc(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts,
+                          ftr_moments* out) {
+  return integrate_impl(c, p, N, seed, opts, out, true);
 }
 
 int ftr_finalize(const ftr_problem* p, const ftr_moments* m, double* res) {

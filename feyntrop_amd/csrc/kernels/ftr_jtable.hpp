@@ -288,11 +288,12 @@ __device__ inline void xo_apply_dev(uint64_t (&s)[4], const uint64_t* q) {
   s[3] = acc[3];
 }
 
-extern "C" __global__ void __launch_bounds__(256) ftrj_block_states(const uint64_t* base, const uint64_t* polys,
-                                                                     int ndig, uint64_t nblocks, uint64_t* out) {
+extern "C" __global__ void __launch_bounds__(256)
+    ftrj_block_states(uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3, const uint64_t* polys, int ndig,
+                      uint64_t nblocks, uint64_t* out) {
   const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblocks) return;
-  uint64_t s[4] = {base[0], base[1], base[2], base[3]};
+  uint64_t s[4] = {b0, b1, b2, b3};
   for (int k = 0; k < ndig; k++) {
     const int d = (int)((b >> (8 * k)) & 255);
     if (d) xo_apply_dev(s, polys + 4 * (255 * k + d - 1));

@@ -83,10 +83,14 @@ typedef struct {
   uint64_t samples_per_lane;
 } ftr_moments;
 
-/* Launch options (0 = automatic). */
+/* Launch options.  The kernel variant (J-table placement, block size) is
+ * fixed when the problem is prepared (ftr_set_variant, ftr_autotune change
+ * it); these fields can only confirm it, and a call whose block or
+ * table_mode differs from it fails with FTR_ERR_INVALID. */
 typedef struct {
-  int block;              /* threads per block: 256 or 1024 */
-  int table_mode;         /* -1 auto, 0 LDS, 1 w in LDS + rest global, 2 global */
+  int block;              /* 0 = the compiled one, else must equal it (256, 512 or 1024) */
+  int table_mode;         /* -1 = the compiled one, else must equal it: 0 whole table in LDS,
+                           * 1 w in LDS + rest global, 2 all global, 3 rows in HBM */
   uint64_t lanes;         /* total lanes; 0 = fill the device */
 } ftr_opts;
 
@@ -116,6 +120,22 @@ int ftr_jtable(const ftr_problem* p, double* Jr, double* r, uint8_t* mm, uint8_t
  * samples equals the reference run with one OpenMP thread. */
 int ftr_integrate(ftr_ctx* ctx, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts,
                   ftr_moments* out);
+
+/* ftr_integrate without any device allocation (no reference counterpart;
+ * for callers that cannot call hipMalloc: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  Same kernels, samples and
+ * moments as ftr_integrate (bit for bit at the same launch geometry).  The
+ * block start states, jump polynomials and partial moments live in a fixed
+ * device array of FTR_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; a launch that needs more (32 + 8 (2 + 4K) bytes per block: about
+ * 7000 blocks at K = 16, 37000 at K = 2; only a large opts->lanes asks for
+ * that many) is rejected with FTR_ERR_LIMIT.  Allocates no device or
+ * pinned memory (the problem must have been prepared, which allocates; the
+ * host keeps its cache of jump polynomials).  All calls on one GPU share the
+ * array: calls that overlap on one GPU (other threads, other contexts) race. */
+#define FTR_NOALLOC_SCRATCH_BYTES (4u << 20)
+int ftr_integrate_noalloc(ftr_ctx* ctx, ftr_problem* p, uint64_t N, uint64_t seed, const ftr_opts* opts,
+                          ftr_moments* out);
 
 /* Final estimates from moments like stats.hpp:86-104 (real path: the
  * imaginary parts are exactly 0, as with the reference's zero updates).
