@@ -175,56 +175,112 @@ def _rel_err(p, m):
     return float(np.hypot(r[0, 0, 1], r[0, 1, 1]) / abs(I0)), I0, r
 
 
-def run_learned(eng, name="triple_ladder", steps=400, batch=1 << 18, lr=3e-3):
+def _combined(p, ms):
+    """Moments of several runs added into one ftr_moments (independent
+    streams: the runs use different seeds)."""
+    import feyntrop_amd
+    t = feyntrop_amd.ftr_moments()
+    t.K = ms[0].K
+    for m in ms:
+        for k in range(m.K):
+            for q in range(2):
+                t.n[k][q] += m.n[k][q]
+                t.s1[k][q] += m.s1[k][q]
+                t.s2[k][q] += m.s2[k][q]
+        t.dropped += m.dropped
+    if p.info.path != 2:   # the real path's two zero updates, once
+        for k in range(t.K):
+            t.n[k][1] = 2
+    return t
+
+
+def _run_to(p, fn, target=1e-3, cap_s=90.0, seed0=1000):
+    """Calls fn(seed) (one integration of N samples each, new seed each call)
+    until the combined eps^0 relative error is <= target; returns (wall s,
+    rel. err reached, samples, calls)."""
+    ms, t0 = [], time.perf_counter()
+    while True:
+        ms.append(fn(seed0 + len(ms)))
+        rel, _, _ = _rel_err(p, _combined(p, ms))
+        wall = time.perf_counter() - t0
+        if rel <= target or wall > cap_s:
+            return wall, rel, sum(int(m.n[0][0]) + m.dropped for m in ms), len(ms)
+
+
+def run_learned(eng, name="triple_ladder", steps=400, batch=1 << 18, lr=3e-3, seeds=3):
     """Learned proposal against the plain sampler on the 3-loop ladder: the
     network is trained on steps x batch (about 1e8) proposal samples, plain
-    sampling judges it (select), then both integrate N = 1e8 samples.
-    time-to-1e-3 = (rel. err at N / 1e-3)^2 x wall time at N (+ training for
-    the learned proposal); equal wall-clock: the learned error scaled to the
-    plain run's wall time."""
+    sampling judges it (select), then both integrate N = 1e8 samples with
+    `seeds` seeds each.  time-to-1e-3 is reported two ways:
+      direct:    wall clock of repeated N-sample calls (new seed each) until
+                 the combined rel. error of eps^0 is <= 1e-3 (the learned
+                 proposal's time includes its training);
+      median:    over the seeds, (rel. err at N / 1e-3)^2 x wall time at N
+                 (+ training for the learned proposal).
+    Equal wall-clock: the plain error at the learned proposal's median time
+    to 1e-3."""
     import feyntrop_amd
     from feyntrop_amd import workloads as W
     prob = W.all_problems()[name]()
     N = int(prob["N"])
     p = eng.prepare(prob)
     eng.integrate(p, 1_000_000, 1)
-    t0 = time.perf_counter()
-    mp = eng.integrate(p, N, 11)
-    tp = time.perf_counter() - t0
-    rel_p, Ip, rp = _rel_err(p, mp)
+    plain = []
+    for s in range(seeds):
+        t0 = time.perf_counter()
+        m = eng.integrate(p, N, 11 + 17 * s)
+        plain.append((time.perf_counter() - t0, m))
     L = feyntrop_amd.Learner(eng, p, init_seed=1)
     t0 = time.perf_counter()
     st = L.train(steps, batch, seed=100, lr=lr)
     t_train = time.perf_counter() - t0
     use, eff_p, eff_l = L.select(n_probe=4_000_000, seed=7)
     L.integrate(1_000_000, 2)
-    t0 = time.perf_counter()
-    ml = L.integrate(N, 12)
-    tl = time.perf_counter() - t0
-    rel_l, Il, rl = _rel_err(p, ml)
-    t13_p = (rel_p / 1e-3) ** 2 * tp
-    t13_l = (rel_l / 1e-3) ** 2 * tl + t_train
-    # equal wall-clock: the plain sampler given the learned run's time to 1e-3
-    rel_p_eq = 1e-3 * np.sqrt(t13_p / t13_l)
+    learned = []
+    for s in range(seeds):
+        t0 = time.perf_counter()
+        m = L.integrate(N, 12 + 17 * s)
+        learned.append((time.perf_counter() - t0, m))
+    rel_p = [_rel_err(p, m)[0] for _, m in plain]
+    rel_l = [_rel_err(p, m)[0] for _, m in learned]
+    t13_p = [(r / 1e-3) ** 2 * t for r, (t, _) in zip(rel_p, plain)]
+    t13_l = [(r / 1e-3) ** 2 * t + t_train for r, (t, _) in zip(rel_l, learned)]
+    ratios = [a / b for a, b in zip(t13_p, t13_l)]
+    # direct: run until 1e-3 (the learned clock starts with its training)
+    wall_p, rel_pd, n_pd, c_pd = _run_to(p, lambda sd: eng.integrate(p, N, sd), seed0=5000)
+    wall_l, rel_ld, n_ld, c_ld = _run_to(p, lambda sd: L.integrate(N, sd), seed0=6000)
+    wall_l += t_train
+    Ip = _rel_err(p, _combined(p, [m for _, m in plain]))
+    Il = _rel_err(p, _combined(p, [m for _, m in learned]))
+    rp, rl = Ip[2], Il[2]
     z = [abs(rl[k, q, 0] - rp[k, q, 0]) / max(np.hypot(rl[k, q, 1], rp[k, q, 1]), 1e-300)
          for k in range(p.K) for q in range(2)]
     fps = flops_per_sample(p.info)
-    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}, N={N:.0e} each, lambda={prob['lambda']}",
+    kp = float(np.mean([m.kernel_ms for _, m in plain]))
+    kl = float(np.mean([m.kernel_ms for _, m in learned]))
+    med_p, med_l = float(np.median(t13_p)), float(np.median(t13_l))
+    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}, N={N:.0e} per call, lambda={prob['lambda']}",
             "network": "MLP 32-128-128-128-32 SiLU, BF16 MFMA (f32 accumulate), Adam",
             "train_samples": steps * batch, "train_s": t_train,
             "train_rel2_first_last": [float(st[0, 0]), float(st[-1, 0])],
             "selected": "learned" if use else "plain",
-            "plain": {"samples_per_s": N / tp, "rel_err": rel_p, "time_to_1e-3_s": t13_p,
-                      "kernel_ms": mp.kernel_ms,
-                      "roofline_frac": fps * N / (mp.kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
-            "learned": {"samples_per_s": N / tl, "rel_err": rel_l, "time_to_1e-3_s": t13_l,
-                        "kernel_ms": ml.kernel_ms,
-                        "roofline_frac": fps * N / (ml.kernel_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
-            "variance_gain": (rel_p ** 2) / (rel_l ** 2),
-            "speedup_time_to_1e-3": t13_p / t13_l,
-            "rel_err_at_equal_wallclock": {"wall_s": t13_l, "plain": rel_p_eq, "learned": 1e-3},
+            "seeds": seeds,
+            "plain": {"samples_per_s": N / float(np.mean([t for t, _ in plain])), "rel_err": rel_p,
+                      "time_to_1e-3_s": t13_p, "kernel_ms": kp,
+                      "roofline_frac": fps * N / (kp * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+            "learned": {"samples_per_s": N / float(np.mean([t for t, _ in learned])), "rel_err": rel_l,
+                        "time_to_1e-3_s": t13_l, "kernel_ms": kl,
+                        "roofline_frac": fps * N / (kl * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+            "variance_gain_median": float(np.median([(a / b) ** 2 for a, b in zip(rel_p, rel_l)])),
+            "speedup_time_to_1e-3_median": float(np.median(ratios)),
+            "speedup_time_to_1e-3_per_seed": ratios,
+            "direct_time_to_1e-3": {"plain_s": wall_p, "plain_rel_err": rel_pd, "plain_samples": n_pd,
+                                    "learned_s_incl_training": wall_l, "learned_rel_err": rel_ld,
+                                    "learned_samples": n_ld, "speedup": wall_p / wall_l},
+            "rel_err_at_equal_wallclock": {"wall_s": med_l, "plain": 1e-3 * np.sqrt(med_p / med_l),
+                                           "learned": 1e-3},
             "max_sigma_plain_vs_learned": float(max(z)),
-            "eps0": {"plain": [Ip.real, Ip.imag], "learned": [Il.real, Il.imag]}}
+            "eps0": {"plain": [Ip[1].real, Ip[1].imag], "learned": [Il[1].real, Il[1].imag]}}
 
 
 def calibration_copy(nbytes):
@@ -276,8 +332,9 @@ def main():
     N = int(prob["N"])
     eng = feyntrop_amd.Engine(0)
     p = eng.prepare(prob)
-    # kernel choice (preparation, untimed): the generic kernel or the one
-    # specialised to this kinematic point, whichever is faster on a probe
+    # kernel choice (preparation, untimed): the generic kernel (the one the
+    # CLI runs) or the one specialised to this kinematic point, whichever is
+    # faster on a probe; the other one is timed after the main line
     probe = {}
     for spec in (False, True):
         eng.set_specialized(p, spec)
@@ -318,13 +375,26 @@ def main():
         "time_to_1e-3_s": t_1e3,
         "rel_err_eps0_at_N": rel_at_N,
         "eps0": [I0.real, I0.imag],
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "bound_note": "FP64 vector ALU issue (per-sample matrices of order <= 9: no MFMA or HBM "
+                                   "work); peak = MI355X FP64 78.6 TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": PMC_TRAFFIC_BYTES.get((args.workload, "specialised" if eng.specialized(p) else
                                                        "generic")) if N == 10**8 else None,
                      "kernel": "ftrk_int", "flops_per_sample": fps, "kernel_ms_mean": mean_kms},
         "learned": None,
     }
+    # the other kernel kind at full N (the CLI runs the generic one)
+    other = not eng.specialized(p)
+    eng.set_specialized(p, other)
+    eng.integrate(p, N, 2)
+    oms = [eng.integrate(p, N, 2000 + i).kernel_ms for i in range(3)]
+    eng.set_specialized(p, not other)
+    okms = float(np.mean(oms))
+    line["roofline"]["other_kernel"] = {
+        "kernel": "specialised to the kinematic point" if other else "generic (the CLI's)",
+        "kernel_ms_mean": okms, "samples_per_s_kernel": N / (okms * 1e-3),
+        "frac": fps * N / (okms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
     if not args.no_learned:
         try:
             line["learned"] = run_learned(eng)
@@ -343,9 +413,16 @@ def main():
                 extras[key] = {"error": str(e)[:300]}
         line["extra_workloads"] = extras
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every core this process may run on, capped by OMP_NUM_THREADS when
+        # the environment sets it (the host's share of cores for this GPU)
+        aff = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS")
+        threads = args.cpu_threads or (min(aff, int(omp)) if omp and omp.isdigit() else aff)
         ns = args.cpu_samples or 2.5e6 * threads
         line["cpu_baseline"] = cpu_baseline(prob, ns, threads)
+        if line["cpu_baseline"]:
+            line["cpu_baseline"].update({"affinity_cores": aff, "nproc": os.cpu_count(),
+                                         "omp_num_threads_env": omp})
         # the same reference binary on one core (bounded: 2e6 samples)
         line["cpu_baseline_1core"] = cpu_baseline(prob, 2e6, 1)
     print(json.dumps(line), flush=True)

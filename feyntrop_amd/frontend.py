@@ -22,6 +22,7 @@ Differences, all additions:
 Reference: py_feyntrop.py:19-298.
 """
 
+import math
 import os
 import sys
 
@@ -126,11 +127,14 @@ def eps_expansion(res, edges, D0):
 
 
 def _round_pair(value, error):
-    """Value and error as strings with two significant digits of the error."""
+    """Value and error as strings, as the reference prints them
+    (py_feyntrop.py:51-63): the number of decimals is |trunc(log10 error)| + 2
+    (so 3.4 -> '3.40', 25 -> '25.000', 0.001 -> '0.00100'), both numbers
+    rounded to it first; an exact zero error prints both as str()."""
     if error == 0:
         return str(value), str(error)
-    digits = max(0, -int(f"{error:e}".split("e")[1]) + 1)
-    return f"{value:.{digits}f}", f"{error:.{digits}f}"
+    decimals = abs(int(math.log10(error))) + 2
+    return f"{round(value, decimals):.{decimals}f}", f"{round(error, decimals):.{decimals}f}"
 
 
 def print_res(res):

@@ -138,6 +138,18 @@ def test_print_res_rounds_to_error(capsys):
     assert "-274.3912" in out[1] and "0.0055" in out[1]
 
 
+@pytest.mark.parametrize("value,error,expect", [
+    (12.3456, 3.4, ("12.35", "3.40")),          # error >= 1: |trunc(log10)| + 2 decimals
+    (1234.56789, 25.0, ("1234.568", "25.000")),
+    (0.123456, 0.001, ("0.12346", "0.00100")),   # exact power of ten
+    (-46.5647, 0.1312, ("-46.56", "0.13")),
+    (5.0, 0.0, ("5.0", "0.0")),
+])
+def test_round_pair_is_the_reference_rule(value, error, expect):
+    """py_feyntrop.py:51-63: decimals = |int(log10(err))| + 2, both rounded."""
+    assert FE._round_pair(value, error) == expect
+
+
 def test_missing_rule_is_an_error():
     edges = [((0, 1), 1, 'm'), ((1, 2), 1, 'm'), ((2, 0), 1, 'm')]
     with pytest.raises(ValueError):
