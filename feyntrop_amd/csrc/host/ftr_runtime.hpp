@@ -127,9 +127,14 @@ void ensure_scratch(ftr_ctx* c, size_t bytes);
 void prep_from_input(ftrh::Prep& pr, const ftr_input* in);
 Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit = (size_t)128 << 30);
 size_t variant_lds(const Variant& v, int E);
-/* the variant keeps the deformation's second derivatives in LDS (FTR_DD_LDS_MAX_E) */
+/* the variant keeps the deformation's second derivatives in LDS: every
+ * deformed-path row-table kernel with blocks of at most 256 threads and
+ * E <= FTR_DD_LDS_MAX_E (E(E+1)/2 doubles per thread; 110 KiB per block at
+ * E = 10).  FEYNTROP_AMD_DDL=0 keeps them in registers instead. */
+int dd_lds_override();
 inline bool dd_in_lds(const Variant& v, int E) {
-  return v.mode == 2 && v.tmode == 3 && v.cparams && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
+  if (dd_lds_override() == 0) return false;
+  return v.mode == 2 && v.tmode == 3 && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
 }
 int fail(ftr_ctx* c, int code, const std::string& msg);
 /* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */

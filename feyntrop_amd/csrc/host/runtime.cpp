@@ -122,6 +122,14 @@ size_t variant_lds(const Variant& v, int E) {
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
 }
 
+int dd_lds_override() {
+  static const int o = [] {
+    const char* e = getenv("FEYNTROP_AMD_DDL");
+    return e ? atoi(e) : -1;
+  }();
+  return o;
+}
+
 static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool learned = false) {
   const ftrh::Graph& g = pr.g;
   /* elimination order: internal vertices, then those with nonzero P rows */
@@ -166,6 +174,7 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
   }
   s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
   s << "  static constexpr bool DDL = " << (dd_in_lds(v, g.E) ? "true" : "false") << ";\n";
+  s << "  static constexpr int BLK = " << v.block << ";\n";
   /* in-step row prefetch of the sampler (ftr_device.hpp sample_point):
    * generic deformed-path kernels */
   s << "  static constexpr bool PF = " << ((v.mode == 2 && !v.cparams) ? "true" : "false") << ";\n";

@@ -15,12 +15,15 @@ typedef __hip_internal::uint64_t uint64_t;
 
 #define FTR_MAX_E 30
 #define FTR_MAX_V 24
-/* Kernels specialised to a kinematic point on the deformed path with the
- * row-table placement, 256-thread blocks and E <= FTR_DD_LDS_MAX_E keep the
- * second derivatives of the deformation (E(E+1)/2 doubles per thread) in LDS
- * instead of registers (measured: 12.1 -> 11.6 ms per 1e8 double-box
- * samples; slower at E = 10 and for the generic kernel). */
-#define FTR_DD_LDS_MAX_E 8
+/* Deformed-path kernels with the row-table placement, blocks of at most 256
+ * threads and E <= FTR_DD_LDS_MAX_E keep the second derivatives of the
+ * deformation (E(E+1)/2 doubles per thread) in LDS, laid out [entry][thread],
+ * instead of registers.  Measured per 1e8 samples against registers: 3-loop
+ * ladder (E = 10) specialised 27.8 -> 22.8 ms, generic 23.6 -> 23.5 ms,
+ * learned 26.9 -> 25.6 ms; double box generic 14.3 -> 13.7 ms (the
+ * 10-edge kernels no longer spill to scratch).  E = 11 would need 132 KiB
+ * per 256-thread block on top of the X slots. */
+#define FTR_DD_LDS_MAX_E 10
 
 namespace ftr {
 

@@ -240,7 +240,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
       g = c;
       if (MM && !mm) lGv = lX;
       MM = mm;
-      tv.xs[e] = XX;
+      tv.xs[e * G::BLK] = XX;
       if (L < oL) {
         lF += lX;
         oL = L;
@@ -252,7 +252,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
       }
     }
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = tv.xs[k];
+    for (int k = 0; k < E; k++) X[k] = tv.xs[k * G::BLK];
     lpsi_tr = lF;
     lG = lGv;
     return;
@@ -325,7 +325,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
     if (MM && !mm) lGv = lX;
     MM = mm;
     if constexpr (TV::TM == 3) {
-      tv.xs[e] = XX;
+      tv.xs[e * G::BLK] = XX;
     } else {
 #pragma unroll
       for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
@@ -342,7 +342,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
   }
   if constexpr (TV::TM == 3) {
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = tv.xs[k];
+    for (int k = 0; k < E; k++) X[k] = tv.xs[k * G::BLK];
   }
   lpsi_tr = lF;
   lG = lGv;
@@ -591,9 +591,10 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     }
     /* first and second derivatives of pphi (laplacian.hpp:102-215) */
     double th[E];
-    /* in registers, or (DDL) in this thread's LDS slots */
+    /* in registers, or (DDL) in this thread's LDS slots (stride G::BLK) */
     double dd_regs[DDL ? 1 : E * (E + 1) / 2];
     double* const dd = DDL ? dd_slots : dd_regs;
+    constexpr int DS = DDL ? G::BLK : 1;
 #pragma unroll
     for (int a = 0; a < E; a++) {
       const int ka = G::pos[G::hi[a]], la = G::pos[G::lo[a]];
@@ -602,14 +603,14 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
       const double xa = Xi[a] * Xi[a];
       const double xas = xa * s;
       th[a] = -xas;
-      dd[FTR_TRI(a, a)] = 2.0 * xas * (Xi[a] - xa * t);
+      dd[FTR_TRI(a, a) * DS] = 2.0 * xas * (Xi[a] - xa * t);
 #pragma unroll
       for (int b = 0; b < a; b++) {
         const int kb = G::pos[G::hi[b]], lb = G::pos[G::lo[b]];
         const double s2 = FTR_SYM(LP, ka, kb) + FTR_SYM(LP, la, lb) - FTR_SYM(LP, ka, lb) - FTR_SYM(LP, la, kb);
         const double t2 = FTR_SYM(Gm, ka, kb) + FTR_SYM(Gm, la, lb) - FTR_SYM(Gm, ka, lb) - FTR_SYM(Gm, la, kb);
         const double xb = Xi[b] * Xi[b];
-        dd[FTR_TRI(a, b)] = -2.0 * xb * xa * s2 * t2;
+        dd[FTR_TRI(a, b) * DS] = -2.0 * xb * xa * s2 * t2;
       }
     }
     /* deformation Jacobian: det(I + i c H X) = det(X^-1 + i c H) prod x.
@@ -625,7 +626,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
         cplx w[E];              /* w_k = d_k L[j][k] */
 #pragma unroll
         for (int k = 0; k < j; k++) w[k] = d[k] * L[FTR_TRI(j, k)];
-        cplx dj = cmk(Xi[j], p.def_ref * dd[FTR_TRI(j, j)]);
+        cplx dj = cmk(Xi[j], p.def_ref * dd[FTR_TRI(j, j) * DS]);
 #pragma unroll
         for (int k = 0; k < j; k++) dj = fms(dj, L[FTR_TRI(j, k)], w[k]);
         d[j] = dj;
@@ -633,7 +634,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
         const cplx inv = crecip(dj);
 #pragma unroll
         for (int i = j + 1; i < E; i++) {
-          cplx v = cmk(0.0, p.def_ref * dd[FTR_TRI(i, j)]);
+          cplx v = cmk(0.0, p.def_ref * dd[FTR_TRI(i, j) * DS]);
 #pragma unroll
           for (int k = 0; k < j; k++) v = fms(v, L[FTR_TRI(i, k)], w[k]);
           L[FTR_TRI(i, j)] = v * inv;
@@ -760,19 +761,21 @@ __host__ __device__ constexpr size_t table_lds_bytes(int E) {
 }
 /* With the tables in HBM (TMODE 3) the LDS holds each thread's E slots of X,
  * where the sampler scatters the radius of the cut edge (one store per cut
- * instead of a select over all E edges); layout [thread][e], so that the
- * reads are one base register plus constant offsets. */
+ * instead of a select over all E edges).  Per-thread LDS slots are laid out
+ * [slot][thread] (slot e of this thread at ptr[e * G::BLK], G::BLK = the
+ * block size): the 64 lanes of a wave touch 64 consecutive doubles whatever
+ * slot each lane picks, so the accesses are free of bank conflicts, and
+ * every offset is a compile-time constant. */
 template <int TMODE>
 __device__ inline size_t xs_lds_bytes(int E) {
   return TMODE == 3 ? (size_t)E * blockDim.x * 8 : 0;
 }
 template <int TMODE>
 __device__ inline double* xs_slots(char* lds, int E) {
-  return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + (size_t)threadIdx.x * E
-                     : nullptr;
+  return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + threadIdx.x : nullptr;
 }
 /* the deformation's second derivatives in LDS (G::DDL): E(E+1)/2 slots per
- * thread after the X slots, layout [thread][entry] */
+ * thread after the X slots, same [slot][thread] layout */
 template <class G>
 __device__ inline size_t dd_lds_bytes() {
   return G::DDL ? (size_t)(G::E * (G::E + 1) / 2) * blockDim.x * 8 : 0;
@@ -780,7 +783,7 @@ __device__ inline size_t dd_lds_bytes() {
 template <class G, int TMODE>
 __device__ inline double* dd_slots(char* lds) {
   return G::DDL ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(G::E) + xs_lds_bytes<TMODE>(G::E)) +
-                      (size_t)threadIdx.x * (G::E * (G::E + 1) / 2)
+                      threadIdx.x
                 : nullptr;
 }
 /* start of the block reduction's LDS area */
@@ -1025,7 +1028,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     if (MM && !mm) lGv = lX;
     MM = mm;
     if (xs) {
-      xs[e] = XX;
+      xs[e * G::BLK] = XX;
     } else {
 #pragma unroll
       for (int k = 0; k < E; k++) X[k] = (k == e) ? XX : X[k];
@@ -1047,7 +1050,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
   }
   if (xs) {
 #pragma unroll
-    for (int k = 0; k < E; k++) X[k] = xs[k];
+    for (int k = 0; k < E; k++) X[k] = xs[k * G::BLK];
   }
   lpsi = lF;
   lG = lGv;
