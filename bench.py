@@ -326,7 +326,7 @@ def main():
     import feyntrop_amd
     from feyntrop_amd import workloads as W
 
-    prob = W.all_problems()[args.workload]()
+    prob = (W.all_problems().get(args.workload) or W.LARGE[args.workload])()
     if args.N:
         prob["N"] = int(args.N)
     N = int(prob["N"])
