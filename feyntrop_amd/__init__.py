@@ -84,6 +84,7 @@ def lib():
     L.ftr_problem_info.argtypes = [vp, C.POINTER(ftr_info)]
     L.ftr_jtable.argtypes = [vp, vp, vp, vp, vp]
     L.ftr_integrate.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts), C.POINTER(ftr_moments)]
+    L.ftr_precompile_batch.argtypes = [C.c_int, C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_integrate_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.POINTER(ftr_opts),
                                         C.POINTER(ftr_moments)]
     L.ftr_finalize.argtypes = [vp, C.POINTER(ftr_moments), vp]
@@ -154,6 +155,17 @@ def precompile(prob):
     inp = _Input(prob)
     err = C.create_string_buffer(8192)
     rc = lib().ftr_precompile(C.byref(inp.s), err, 8192)
+    if rc:
+        raise FeyntropError(rc, err.value.decode(errors="replace"))
+
+
+def precompile_batch(probs):
+    """Compile the bucket kernels of a batched integration of these problems
+    (those the preprocessing accepts) into the kernel cache (no GPU needed)."""
+    inps = [_Input(pr) for pr in probs]
+    arr = (ftr_input * len(inps))(*[x.s for x in inps])
+    err = C.create_string_buffer(8192)
+    rc = lib().ftr_precompile_batch(len(inps), arr, err, 8192)
     if rc:
         raise FeyntropError(rc, err.value.decode(errors="replace"))
 

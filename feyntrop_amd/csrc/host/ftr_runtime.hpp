@@ -36,6 +36,9 @@ struct Module {
   hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr, f_bat = nullptr;
 };
 
+/* concurrent streams of a batch graph's bucket launches */
+constexpr int FTR_SIDE_STREAMS = 4;
+
 struct ftr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -65,6 +68,8 @@ struct ftr_ctx {
   int jtable_mode = 0;        /* 0 auto, 1 host, 2 device (ftr_set_jtable_mode) */
   std::vector<std::string> batch_errors; /* messages of ftr_prepare_batch */
   hipGraphExec_t batch_exec = nullptr;   /* last batch graph, replayed for equal shapes */
+  hipStream_t side[FTR_SIDE_STREAMS] = {}; /* the batch graph's concurrent bucket launches */
+  hipEvent_t ev_fork = nullptr, ev_join[FTR_SIDE_STREAMS] = {};
   std::string batch_key;
   int batch_replays = 0;
 };

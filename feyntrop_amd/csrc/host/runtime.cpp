@@ -130,7 +130,12 @@ int dd_lds_override() {
   return o;
 }
 
-static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool learned = false) {
+/* The graph spec of a problem's kernel (struct `gname`, and `gname`P for the
+ * parameters of a kernel specialised to the kinematic point), followed by the
+ * kernel instantiations unless `instantiate` is false (bucket kernels list
+ * several specs and instantiate them themselves). */
+static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool learned = false,
+                               const std::string& gname = "FG", bool instantiate = true) {
   const ftrh::Graph& g = pr.g;
   /* elimination order: internal vertices, then those with nonzero P rows */
   std::vector<int> ord, pos(g.V);
@@ -162,7 +167,7 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
       for (int i = 0; i < n; i++) s << (i ? "," : "") << lit(a[i]);
       s << "};\n";
     };
-    s << "\nstruct FP {\n";
+    s << "\nstruct " << gname << "P {\n";
     s << "  static constexpr double D = " << lit(d.D) << ", mhalfD = " << lit(d.mhalfD) << ", W = " << lit(d.W)
       << ", mW = " << lit(d.mW) << ", Lf = " << lit(d.Lf) << ", IGtr = " << lit(d.IGtr) << ", def_ref = "
       << lit(d.def_ref) << ", inv_dlw = " << lit(d.inv_dlw) << ", iD = " << lit(d.iD) << ", iW = " << lit(d.iW)
@@ -172,18 +177,20 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
     darr("P", d.P, FTR_MAX_V * FTR_MAX_V);
     s << "};\n";
   }
-  s << "\nstruct FG {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
+  s << "\nstruct " << gname << " {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
   s << "  static constexpr bool DDL = " << (dd_in_lds(v, g.E) ? "true" : "false") << ";\n";
   s << "  static constexpr int BLK = " << v.block << ";\n";
   /* in-step row prefetch of the sampler (ftr_device.hpp sample_point):
    * generic deformed-path kernels */
   s << "  static constexpr bool PF = " << ((v.mode == 2 && !v.cparams) ? "true" : "false") << ";\n";
-  s << (v.cparams ? "  using PRM = FP;\n" : "  using PRM = ftr::DevParams;\n");
+  s << (v.cparams ? "  using PRM = " + gname + "P;\n" : std::string("  using PRM = ftr::DevParams;\n"));
   arr(s, "hi", g.hi);
   arr(s, "lo", g.lo);
   arr(s, "ord", ord);
   arr(s, "pos", pos);
-  s << "};\nFTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
+  s << "};\n";
+  if (!instantiate) return s.str();
+  s << "FTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
   if (learned)
     s << "FTR_INSTANTIATE_LEARNED(ftrl, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
   return s.str();
@@ -574,6 +581,11 @@ void ftr_destroy(ftr_ctx* c) {
   if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
   if (c->d_batch) (void)hipFree(c->d_batch);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  for (int k = 0; k < FTR_SIDE_STREAMS; k++) {
+    if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1030,10 +1042,186 @@ int ftr_replay(ftr_ctx* c, ftr_problem* p, uint64_t n, const double* X_dev, cons
   return FTR_OK;
 }
 
-/* Batched scan: problems sharing a kernel (same graph spec, K, path and
- * variant) form a group served by one launch of the multi-problem kernel;
- * every block works on one problem.  All groups and the copy-back of the
- * partial moments are captured in one hipGraph and launched once. */
+/* Batched scan: one persistent launch per bucket of problems.
+ *
+ * A bucket gathers the problems whose kernels share (path, eps orders, table
+ * placement, block size) and the graph size (E, V).  Its kernel is generated from the graph specs of all of
+ * its topologies (each one compile-time specialised, dispatched by a switch)
+ * and runs a grid sized to the device; the blocks take units of work --
+ * (problem, virtual block) -- from an atomic counter, the costliest first,
+ * until the list is exhausted.  Every unit is the block of one-call
+ * integration it stands for: the same stream positions and the same
+ * partial moments, so every problem's result is that of its own call up to
+ * the summation order.  The counter reset, all bucket kernels and the
+ * copy-back of the partials are captured in one hipGraph; a call with the
+ * same launch shapes replays it. */
+namespace {
+
+/* size key of a bucket: the edge and vertex counts, so that the topologies
+ * that share a bucket kernel have the same matrix sizes and register and LDS
+ * demand (a kernel's occupancy is that of its largest case) */
+int size_class(const ftrh::Graph& g) { return g.E * 64 + g.V; }
+
+/* relative cost of one sample (ordering of the work list only): the
+ * algorithmic FP64 flops plus the draws and transcendentals of the sampler */
+double sample_cost(const ftrh::Prep& pr) {
+  const double E = pr.g.E, n = pr.g.V - 1, K = pr.K;
+  double f = E * (E + 1) + 4 * E + 40 * (2 * E - 1);
+  if (pr.path == 2)
+    f += n * n * n / 3 + 2 * n * n * n + 4 * n * n * n + 5.5 * E * E + 8.0 / 3 * n * n * n + 8 * n * n * n +
+         8.0 / 3 * E * E * E + 2 * E * E + 40 * E + 8 * K;
+  else
+    f += n * n * n / 3 + 2 * n * n * n + n * n + (n + 1) * (n + 1) + 12 * E + 3 * K;
+  return f;
+}
+
+struct Bucket {
+  int mode = 0, K = 1, tmode = 0, block = 256;
+  std::vector<int> idx;                  /* problems with N > 0 */
+  std::vector<int> topo;                 /* case of each problem in the bucket kernel */
+  std::vector<std::string> specs;        /* one graph spec per case */
+  std::vector<std::pair<int, size_t>> polys; /* (E, offset in the polynomial area) */
+  hipFunction_t f = nullptr;
+  size_t lds = 0;
+  uint64_t S = 0;
+  uint32_t grid = 0, nwork = 0;
+  size_t item_off = 0, work_off = 0;
+};
+
+}  // namespace
+
+/* Buckets of a batch: members[i] = (preparation, generic variant) of every
+ * problem with N > 0, in order; the cases of a bucket are numbered in order
+ * of first appearance.  lds_of[i]: the member's LDS bytes. */
+static std::vector<Bucket> plan_buckets(const std::vector<std::pair<const ftrh::Prep*, Variant>>& members,
+                                        const std::vector<int>& index, std::vector<size_t>& lds_of) {
+  std::map<std::tuple<int, int, int, int, int>, Bucket> bmap;
+  lds_of.assign(members.size(), 0);
+  for (size_t m = 0; m < members.size(); m++) {
+    const ftrh::Prep& pr = *members[m].first;
+    const Variant& v = members[m].second;
+    Bucket& b = bmap[std::make_tuple(v.mode, v.K, v.tmode, v.block, size_class(pr.g))];
+    b.mode = v.mode;
+    b.K = v.K;
+    b.tmode = v.tmode;
+    b.block = v.block;
+    const std::string spec = spec_source(pr, v, false, "@", false);
+    int t = -1;
+    for (size_t q = 0; q < b.specs.size(); q++)
+      if (b.specs[q] == spec) t = (int)q;
+    if (t < 0) {
+      t = (int)b.specs.size();
+      b.specs.push_back(spec);
+    }
+    b.idx.push_back(index[m]);
+    b.topo.push_back(t);
+    lds_of[m] = variant_lds(v, pr.g.E);
+    b.lds = std::max(b.lds, lds_of[m]);
+  }
+  std::vector<Bucket> bs;
+  for (auto& kv : bmap) {
+    Bucket b = std::move(kv.second);
+    /* canonical case order (sorted specs): a set of topologies always gives
+     * the same kernel source, whichever problems brought them */
+    {
+      std::vector<int> perm(b.specs.size());
+      for (size_t t = 0; t < perm.size(); t++) perm[t] = (int)t;
+      std::sort(perm.begin(), perm.end(), [&](int x, int y) { return b.specs[x] < b.specs[y]; });
+      std::vector<int> rank(perm.size());
+      std::vector<std::string> sorted(perm.size());
+      for (size_t r = 0; r < perm.size(); r++) {
+        rank[perm[r]] = (int)r;
+        sorted[r] = b.specs[perm[r]];
+      }
+      b.specs = std::move(sorted);
+      for (int& t : b.topo) t = rank[t];
+    }
+    for (size_t t = 0; t < b.specs.size(); t++) {
+      std::string& sp = b.specs[t];
+      const std::string nm = "FG" + std::to_string(t);
+      for (size_t pos; (pos = sp.find('@')) != std::string::npos;) sp.replace(pos, 1, nm);
+    }
+    bs.push_back(std::move(b));
+  }
+  return bs;
+}
+
+/* kernel source of a bucket: every case's graph spec and the persistent loop */
+static std::string bucket_source(const Bucket& b) {
+  std::ostringstream src;
+  src << ftr_embedded_device_source;
+  for (size_t t = 0; t < b.specs.size(); t++) src << b.specs[t];
+  src << "\nFTR_BUCKET_BEGIN(ftrb_run, " << b.block << ")\n";
+  for (size_t t = 0; t < b.specs.size(); t++)
+    src << "FTR_BUCKET_CASE(" << t << ", FG" << t << ", " << b.K << ", " << b.mode << ", " << b.tmode << ")\n";
+  src << "FTR_BUCKET_END\n";
+  return src.str();
+}
+
+/* side streams and events of the batch graph's fork and join (created once
+ * per context, default priority) */
+static void ensure_side_streams(ftr_ctx* c) {
+  if (c->side[0]) return;
+  for (int k = 0; k < FTR_SIDE_STREAMS; k++) {
+    HIPCHK(hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+}
+
+static hipFunction_t bucket_kernel(ftr_ctx* c, const Bucket& b) {
+  hipModule_t mod = load_module_raw(c, bucket_source(b));
+  hipFunction_t f = nullptr;
+  HIPCHK(hipModuleGetFunction(&f, mod, "ftrb_run"));
+  if (b.lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds);
+  return f;
+}
+
+int ftr_precompile_batch(int n, const ftr_input* in, char* err, int errlen) {
+  try {
+    if (n < 0 || (n > 0 && !in)) throw Error(FTR_ERR_INVALID, "invalid batch");
+    std::vector<ftrh::Prep> prs(n);
+    std::vector<char> ok(n, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; i++) {
+      try {
+        prep_from_input(prs[i], &in[i]);
+        ok[i] = 1;
+      } catch (std::exception&) {
+        ok[i] = 0; /* rejected problems are not part of a batch */
+      }
+    }
+    /* both LDS limits a gfx950 part may report */
+    for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
+      std::vector<std::pair<const ftrh::Prep*, Variant>> members;
+      std::vector<int> index;
+      for (int i = 0; i < n; i++) {
+        if (!ok[i]) continue;
+        Variant v = choose_variant(prs[i], lim);
+        v.cparams = 0;
+        members.emplace_back(&prs[i], v);
+        index.push_back(i);
+      }
+      std::vector<size_t> lds_of;
+      std::vector<Bucket> bs = plan_buckets(members, index, lds_of);
+      std::vector<std::string> srcs;
+      for (auto& b : bs) srcs.push_back(bucket_source(b));
+#pragma omp parallel for schedule(dynamic, 1)
+      for (long q = 0; q < (long)srcs.size(); q++) code_object(srcs[q], nullptr);
+    }
+    code_object(ftr_embedded_jtable_source, nullptr);
+  } catch (std::exception& e) {
+    if (err && errlen > 0) {
+      strncpy(err, e.what(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fe ? fe->code : FTR_ERR_INVALID;
+  }
+  return FTR_OK;
+}
+
 int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint64_t* Ns, const uint64_t* seeds,
                         ftr_moments* out) {
   if (!c || n < 0 || (n > 0 && (!probs || !Ns || !seeds || !out))) return FTR_ERR_INVALID;
@@ -1041,67 +1229,67 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
     const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(c->device));
     for (int i = 0; i < n; i++) {
-      if (!probs[i] || probs[i]->ctx != c || !probs[i]->m.f_bat)
+      if (!probs[i] || probs[i]->ctx != c || !probs[i]->m.f_int)
         throw Error(FTR_ERR_INVALID, "batch: every problem must be prepared on this context");
       memset(&out[i], 0, sizeof out[i]);
       out[i].K = probs[i]->prep.K;
     }
-    /* groups by kernel module */
-    std::map<hipFunction_t, std::vector<int>> groups;
-    for (int i = 0; i < n; i++)
-      if (Ns[i] > 0) groups[probs[i]->m.f_bat].push_back(i);
-    struct G {
-      hipFunction_t f;
-      int block;
-      size_t lds;
-      uint64_t S;
-      std::vector<int> idx;
-      size_t item_off, poly_off;
-      uint32_t nblocks;
-    };
-    std::vector<G> gs;
-    size_t n_items = 0, n_states = 0, n_polys = 0, n_part = 0;
+    /* buckets; within a bucket one case per distinct graph spec */
+    std::vector<std::pair<const ftrh::Prep*, Variant>> members;
+    std::vector<int> index;
+    for (int i = 0; i < n; i++) {
+      if (Ns[i] == 0) continue;
+      Variant v = probs[i]->var;
+      v.cparams = 0; /* bucket kernels read the parameters from memory */
+      members.emplace_back(&probs[i]->prep, v);
+      index.push_back(i);
+    }
+    std::vector<size_t> lds_of;
+    std::vector<Bucket> bs = plan_buckets(members, index, lds_of);
+    for (auto& b : bs) b.f = bucket_kernel(c, b); /* loaded once per context and bucket composition */
+    size_t n_items = 0, n_states = 0, n_polys = 0, n_part = 0, n_work = 0;
     std::vector<size_t> state_off(n, 0), part_off(n, 0);
     std::vector<uint32_t> nb(n, 0);
-    for (auto& kv : groups) {
-      G g;
-      g.f = kv.first;
-      g.idx = kv.second;
-      ftr_problem* p0 = probs[g.idx[0]];
-      g.block = p0->var.block;
-      g.lds = p0->lds;
+    for (auto& b : bs) {
       int nbpc = 0;
-      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, g.f, g.block, g.lds) != hipSuccess || nbpc < 1)
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, b.f, b.block, b.lds) != hipSuccess || nbpc < 1)
         nbpc = 1;
-      const uint64_t lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * g.block;
+      const uint64_t lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * b.block;
       uint64_t tot = 0;
-      for (int i : g.idx) tot += Ns[i];
+      for (int i : b.idx) tot += Ns[i];
       /* enough samples per lane to amortise the stream positioning, enough
        * lanes for every problem */
-      g.S = std::max<uint64_t>(16, (tot + lanes_target - 1) / lanes_target);
-      g.item_off = n_items;
-      n_items += g.idx.size();
+      b.S = std::max<uint64_t>(16, (tot + lanes_target - 1) / lanes_target);
+      b.item_off = n_items;
+      n_items += b.idx.size();
       int lb = 0;
-      while ((1 << lb) < g.block) lb++;
-      g.poly_off = n_polys;
-      n_polys += std::max<size_t>(lane_poly_count(lb), 1);
-      g.nblocks = 0;
-      for (int i : g.idx) {
-        const uint64_t lanes = (Ns[i] + g.S - 1) / g.S;
-        nb[i] = (uint32_t)((lanes + g.block - 1) / g.block);
+      while ((1 << lb) < b.block) lb++;
+      for (int i : b.idx) {
+        const int E = probs[i]->prep.g.E;
+        bool have = false;
+        for (auto& pe : b.polys) have = have || pe.first == E;
+        if (!have) {
+          b.polys.emplace_back(E, n_polys);
+          n_polys += std::max<size_t>(lane_poly_count(lb), 1);
+        }
+        const uint64_t lanes = (Ns[i] + b.S - 1) / b.S;
+        nb[i] = (uint32_t)((lanes + b.block - 1) / b.block);
         state_off[i] = n_states;
         n_states += nb[i];
         part_off[i] = n_part;
         n_part += (size_t)nb[i] * (2 + 4 * probs[i]->prep.K);
-        g.nblocks += nb[i];
+        b.nwork += nb[i];
       }
-      gs.push_back(g);
+      b.work_off = n_work;
+      n_work += b.nwork;
+      b.grid = (uint32_t)std::min<uint64_t>((uint64_t)c->prop.multiProcessorCount * nbpc, b.nwork);
     }
-    /* device layout: items | states | polys | partials */
+    /* device layout: items | work | counters | states | polys | partials */
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t b_items = al(n_items * sizeof(ftr::BatchItem)), b_states = al(n_states * 32),
-                 b_polys = al(n_polys * 32 + 32), b_part = al(n_part * 8 + 8);
-    const size_t need = b_items + b_states + b_polys + b_part;
+    const size_t b_items = al(n_items * sizeof(ftr::BatchItem)), b_work = al(n_work * sizeof(ftr::BatchWork) + 16),
+                 b_cnt = al(bs.size() * 4 + 4), b_states = al(n_states * 32), b_polys = al(n_polys * 32 + 32),
+                 b_part = al(n_part * 8 + 8);
+    const size_t need = b_items + b_work + b_cnt + b_states + b_polys + b_part;
     if (need > c->batch_bytes) {
       if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
       c->batch_exec = nullptr;
@@ -1112,50 +1300,77 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
     }
     char* base = (char*)c->d_batch;
     ftr::BatchItem* d_items = (ftr::BatchItem*)base;
-    uint64_t* d_states = (uint64_t*)(base + b_items);
-    uint64_t* d_polys = (uint64_t*)(base + b_items + b_states);
-    double* d_part = (double*)(base + b_items + b_states + b_polys);
+    ftr::BatchWork* d_work = (ftr::BatchWork*)(base + b_items);
+    uint32_t* d_cnt = (uint32_t*)(base + b_items + b_work);
+    uint64_t* d_states = (uint64_t*)(base + b_items + b_work + b_cnt);
+    uint64_t* d_polys = (uint64_t*)(base + b_items + b_work + b_cnt + b_states);
+    double* d_part = (double*)(base + b_items + b_work + b_cnt + b_states + b_polys);
     std::vector<ftr::BatchItem> items(n_items);
+    std::vector<ftr::BatchWork> work(n_work);
     std::vector<uint64_t> states(4 * n_states), polys(4 * n_polys, 0);
-    for (auto& g : gs) {
-      const int E = probs[g.idx[0]]->prep.g.E;
-      const uint64_t per = 2 * (uint64_t)E - 1;
-      const unsigned __int128 dl = (unsigned __int128)g.S * per;
+    for (auto& b : bs) {
       int lb = 0;
-      while ((1 << lb) < g.block) lb++;
-      /* one block: S * per * block */
-      const ftrh::Poly qb =
-          lane_digit_polys(ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64)), lb, &polys[4 * g.poly_off]);
-      uint32_t b0 = 0;
-      for (size_t t = 0; t < g.idx.size(); t++) {
-        const int i = g.idx[t];
+      while ((1 << lb) < b.block) lb++;
+      /* lane polynomials per edge count, and the jump of one block */
+      std::map<int, ftrh::Poly> qblock;
+      std::map<int, size_t> poff;
+      for (auto& pe : b.polys) {
+        const uint64_t per = 2 * (uint64_t)pe.first - 1;
+        const unsigned __int128 dl = (unsigned __int128)b.S * per;
+        qblock[pe.first] =
+            lane_digit_polys(ftrh::xo_jump_poly((uint64_t)dl, (uint64_t)(dl >> 64)), lb, &polys[4 * pe.second]);
+        poff[pe.first] = pe.second;
+      }
+      /* the work list, costliest units first */
+      std::vector<std::pair<double, ftr::BatchWork>> units;
+      units.reserve(b.nwork);
+      for (size_t t = 0; t < b.idx.size(); t++) {
+        const int i = b.idx[t];
         ftr_problem* p = probs[i];
-        ftr::BatchItem& it = items[g.item_off + t];
+        ftr::BatchItem& it = items[b.item_off + t];
         memset(&it, 0, sizeof it);
         fill_tables(it.a, p);
         it.a.block_state = d_states + 4 * state_off[i];
-        it.a.lane_poly = d_polys + 4 * g.poly_off;
+        it.a.lane_poly = d_polys + 4 * poff[p->prep.g.E];
         it.a.N = Ns[i];
-        it.a.S = g.S;
+        it.a.S = b.S;
         it.a.partials = d_part + part_off[i];
-        it.block0 = b0;
+        it.block0 = 0;
         it.nblocks = nb[i];
-        b0 += nb[i];
+        const double cs = sample_cost(p->prep);
+        const uint64_t per_block = b.S * (uint64_t)b.block;
+        for (uint32_t k = 0; k < nb[i]; k++) {
+          const uint64_t first = (uint64_t)k * per_block;
+          const uint64_t ns = std::min<uint64_t>(per_block, Ns[i] - first);
+          ftr::BatchWork w;
+          w.item = (uint32_t)(b.item_off + t);
+          w.blk = k;
+          w.topo = (uint32_t)b.topo[t];
+          w.pad = 0;
+          units.emplace_back(cs * (double)ns, w);
+        }
       }
+      std::stable_sort(units.begin(), units.end(),
+                       [](const std::pair<double, ftr::BatchWork>& x, const std::pair<double, ftr::BatchWork>& y) {
+                         return x.first > y.first;
+                       });
+      for (size_t u = 0; u < units.size(); u++) work[b.work_off + u] = units[u].second;
       /* stream start state of every block of every problem */
 #pragma omp parallel for schedule(dynamic, 1)
-      for (long t = 0; t < (long)g.idx.size(); t++) {
-        const int i = g.idx[t];
+      for (long t = 0; t < (long)b.idx.size(); t++) {
+        const int i = b.idx[t];
+        const ftrh::Poly& qb = qblock[probs[i]->prep.g.E];
         uint64_t st[4];
         ftrh::xo_seed(seeds[i], st);
-        for (uint32_t b = 0; b < nb[i]; b++) {
-          memcpy(&states[4 * (state_off[i] + b)], st, 32);
-          if (b + 1 < nb[i]) ftrh::xo_apply(qb, st);
+        for (uint32_t k = 0; k < nb[i]; k++) {
+          memcpy(&states[4 * (state_off[i] + k)], st, 32);
+          if (k + 1 < nb[i]) ftrh::xo_apply(qb, st);
         }
       }
     }
     if (n_items) {
       HIPCHK(hipMemcpyHtoDAsync(d_items, items.data(), items.size() * sizeof(ftr::BatchItem), c->stream));
+      HIPCHK(hipMemcpyHtoDAsync(d_work, work.data(), work.size() * sizeof(ftr::BatchWork), c->stream));
       HIPCHK(hipMemcpyHtoDAsync(d_states, states.data(), states.size() * 8, c->stream));
       HIPCHK(hipMemcpyHtoDAsync(d_polys, polys.data(), polys.size() * 8, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
@@ -1168,26 +1383,51 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
         c->pinned_bytes = n_part * 8;
       }
       double* h_part = c->h_pinned;
-      /* one graph: all group kernels, then the copy-back.  Its nodes depend
-       * only on the launch shapes and buffer addresses (the items, states
-       * and polynomials are uploaded above, outside the graph), so a call
-       * with the same shapes replays the previous graph */
+      /* one graph: the counter reset, every bucket kernel, then the
+       * copy-back.  Its nodes depend only on the launch shapes and buffer
+       * addresses (items, work lists, states and polynomials are uploaded
+       * above, outside the graph), so a call with the same shapes replays
+       * the previous graph */
       std::ostringstream key;
-      key << (const void*)c->d_batch << ' ' << (const void*)h_part << ' ' << n_part;
-      for (auto& g : gs)
-        key << ' ' << (const void*)g.f << ':' << g.nblocks << ':' << g.block << ':' << g.lds << ':' << g.item_off << ':'
-            << g.idx.size();
+      key << (const void*)c->d_batch << ' ' << (const void*)h_part << ' ' << n_part << ' ' << bs.size();
+      for (auto& b : bs)
+        key << ' ' << (const void*)b.f << ':' << b.grid << ':' << b.nwork << ':' << b.block << ':' << b.lds << ':'
+            << b.work_off;
       if (!c->batch_exec || c->batch_key != key.str()) {
         if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
         c->batch_exec = nullptr;
         c->batch_key.clear();
+        hipFunction_t fz = nullptr;
+        {
+          hipModule_t mz = load_module_raw(c, ftr_embedded_jtable_source);
+          HIPCHK(hipModuleGetFunction(&fz, mz, "ftrj_zero_u32"));
+        }
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (auto& g : gs) {
-          const ftr::BatchItem* it = d_items + g.item_off;
-          int ni = (int)g.idx.size();
-          void* args[] = {(void*)&it, &ni};
-          HIPCHK(hipModuleLaunchKernel(g.f, g.nblocks, 1, 1, g.block, 1, 1, (unsigned)g.lds, c->stream, args, nullptr));
+        {
+          int nc = (int)bs.size();
+          void* args[] = {&d_cnt, &nc};
+          HIPCHK(hipModuleLaunchKernel(fz, (unsigned)((nc + 63) / 64), 1, 1, 64, 1, 1, 0, c->stream, args, nullptr));
+        }
+        /* the buckets run concurrently: forked over the side streams
+         * (default priority) and joined before the copy-back, so that one
+         * bucket's tail overlaps the next bucket's work */
+        ensure_side_streams(c);
+        HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+        for (int k = 0; k < FTR_SIDE_STREAMS; k++) HIPCHK(hipStreamWaitEvent(c->side[k], c->ev_fork, 0));
+        for (size_t q = 0; q < bs.size(); q++) {
+          Bucket& b = bs[q];
+          const ftr::BatchItem* it = d_items;
+          const ftr::BatchWork* wk = d_work + b.work_off;
+          uint32_t nw = b.nwork;
+          uint32_t* cnt = d_cnt + q;
+          void* args[] = {(void*)&it, (void*)&wk, &nw, &cnt};
+          HIPCHK(hipModuleLaunchKernel(b.f, b.grid, 1, 1, b.block, 1, 1, (unsigned)b.lds,
+                                       c->side[q % FTR_SIDE_STREAMS], args, nullptr));
+        }
+        for (int k = 0; k < FTR_SIDE_STREAMS; k++) {
+          HIPCHK(hipEventRecord(c->ev_join[k], c->side[k]));
+          HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
         }
         HIPCHK(hipMemcpyDtoHAsync(h_part, d_part, n_part * 8, c->stream));
         HIPCHK(hipStreamEndCapture(c->stream, &graph));
@@ -1204,12 +1444,12 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
       HIPCHK(hipStreamSynchronize(c->stream));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      for (auto& g : gs)
-        for (int i : g.idx) {
+      for (auto& b : bs)
+        for (int i : b.idx) {
           const int K = probs[i]->prep.K, NQ = 2 + 4 * K;
           std::vector<double> tot(NQ, 0.0);
-          for (uint32_t b = 0; b < nb[i]; b++)
-            for (int q = 0; q < NQ; q++) tot[q] += h_part[part_off[i] + (size_t)b * NQ + q];
+          for (uint32_t k = 0; k < nb[i]; k++)
+            for (int q = 0; q < NQ; q++) tot[q] += h_part[part_off[i] + (size_t)k * NQ + q];
           ftr_moments& m = out[i];
           m.dropped = (uint64_t)tot[1];
           for (int k = 0; k < K; k++) {
@@ -1220,8 +1460,8 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
             m.s2[k][1] = tot[5 + 4 * k];
           }
           m.kernel_ms = ms; /* the whole graph */
-          m.lanes = (uint64_t)nb[i] * g.block;
-          m.samples_per_lane = g.S;
+          m.lanes = (uint64_t)nb[i] * b.block;
+          m.samples_per_lane = b.S;
         }
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

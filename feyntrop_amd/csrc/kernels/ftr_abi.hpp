@@ -78,6 +78,17 @@ struct BatchItem {
   uint32_t nblocks;
 };
 
+/* One unit of a persistent batched launch: virtual block `blk` of batch item
+ * `item`, whose graph is case `topo` of the bucket kernel.  A bucket kernel
+ * runs a grid sized to the device; its blocks take units from an atomic
+ * counter until the list is exhausted. */
+struct BatchWork {
+  uint32_t item;
+  uint32_t blk;
+  uint32_t topo;
+  uint32_t pad;
+};
+
 /* Learned proposal tables (2^E states g; E <= 16):
  *   lw[g*E+e]     cut weight of e from g: w(g\e) exp(Delta_e(g))
  *   lZ[g]         sum of lw[g*E+e] over e in g, ascending e

@@ -1402,7 +1402,47 @@ __device__ inline void batch_body(const BatchItem* items, int nitems) {
   integrate_body<G, K, MODE, TMODE>(items[lo].a, b - items[lo].block0);
 }
 
+/* Persistent batched launch (no reference counterpart): the next unit of
+ * work for this block, or nwork when the list is exhausted.  Thread 0 takes
+ * the unit from the counter; the two barriers publish it to the block and
+ * keep the slot from being overwritten before every thread has read it.
+ * Every block leaves the loop once the counter passes nwork, so the grid
+ * drains. */
+__device__ inline uint32_t persistent_next(uint32_t* counter, uint32_t nwork) {
+  __shared__ uint32_t slot;
+  if (threadIdx.x == 0) {
+    const uint32_t w = atomicAdd(counter, 1u);
+    slot = w < nwork ? w : nwork;
+  }
+  __syncthreads();
+  const uint32_t w = slot;
+  __syncthreads();
+  return w;
+}
+
 }  // namespace ftr
+
+/* A bucket kernel: graph specs FG0..FGn-1 sharing (K, MODE, TMODE, BLOCK),
+ * one persistent grid over BatchWork units; the generated source lists one
+ * FTR_BUCKET_CASE per graph between FTR_BUCKET_BEGIN and FTR_BUCKET_END. */
+#define FTR_BUCKET_BEGIN(NAME, BLOCK)                                                                 \
+  extern "C" __global__ void __launch_bounds__(BLOCK)                                                 \
+      NAME(const ftr::BatchItem* items, const ftr::BatchWork* work, uint32_t nwork, uint32_t* counter) { \
+    for (;;) {                                                                                        \
+      const uint32_t w_ = ftr::persistent_next(counter, nwork);                                       \
+      if (w_ >= nwork) return;                                                                        \
+      const ftr::BatchWork wk_ = work[w_];                                                            \
+      switch (wk_.topo) {
+#define FTR_BUCKET_CASE(I, G, K, MODE, TMODE)                                                         \
+  case I:                                                                                             \
+    ftr::integrate_body<G, K, MODE, TMODE>(items[wk_.item].a, wk_.blk);                               \
+    break;
+#define FTR_BUCKET_END                                                                                \
+  default:                                                                                            \
+    break;                                                                                            \
+    }                                                                                                 \
+    }                                                                                                 \
+    }
 
 /* Learned-proposal kernels (compiled only for problems that use them). */
 #define FTR_INSTANTIATE_LEARNED(NAME, G, K, MODE, TMODE, BLOCK)                                   \

@@ -305,6 +305,13 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 
+/* zero n counters (the work counters of persistent batched launches; a
+ * kernel rather than a memset so that the launch graph holds no memset node) */
+extern "C" __global__ void __launch_bounds__(64) ftrj_zero_u32(uint32_t* c, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) c[i] = 0u;
+}
+
 #define FTR_JT_KERNEL(VB)                                                                                       \
   extern "C" __global__ void __launch_bounds__(64)                                                              \
       ftrj_level_##VB(const ftrj::JtParams* prm, int n, uint64_t count, const uint64_t* binom, const double* Xs, \

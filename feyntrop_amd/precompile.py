@@ -22,6 +22,29 @@ def problems():
     return probs
 
 
+def batches():
+    """Problem lists whose batched integration the benchmark and the tests
+    run: the 4096-point scan and its warm-up, the mixed scan checked against
+    the oracle, and the batch CLI cases."""
+    from feyntrop_amd import host_accepts
+    scan = [c for c in workloads.scan_points(8192, seed=0) if host_accepts(c)][:4096]
+    out = [scan, scan[:8]]
+    for seed in (21, 33):
+        cand = workloads.scan_points(1500, seed=seed)
+        seen, pts = set(), []
+        for c in cand:
+            path = host_path(c)
+            if path is not None and (c["_topology"], path) not in seen:
+                seen.add((c["_topology"], path))
+                pts.append(c)
+        out.append(pts)
+    for seed, n in ((9, 12), (7, 24), (11, 12)):
+        pts = workloads.scan_points(n, seed=seed, accept=host_accepts)
+        out.append(pts)
+        out.append(pts[1:])
+    return out
+
+
 def _one(item):
     name, fn, learned = item
     try:
@@ -48,6 +71,14 @@ def main():
                 bad += 1
                 print(f"precompile {name}: {err}", file=sys.stderr)
     print(f"precompiled {len(items) - bad}/{len(items)} problems")
+    from feyntrop_amd import precompile_batch
+    for b in batches():
+        try:
+            precompile_batch(b)
+        except Exception as e:  # report, keep going
+            bad += 1
+            print(f"precompile batch of {len(b)}: {e}", file=sys.stderr)
+    print("precompiled the bucket kernels of the benchmark's and the tests' batches")
     return 1 if bad else 0
 
 

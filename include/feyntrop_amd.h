@@ -155,12 +155,21 @@ int ftr_replay(ftr_ctx* ctx, ftr_problem* p, uint64_t n, const double* X_dev,
 int ftr_run_json(ftr_ctx* ctx, const char* json_in, char** json_out, char** log_out);
 void ftr_free_string(char* s);
 
-/* Batched scan (no reference counterpart): n problems in one hipGraph (one
- * multi-problem kernel per group of problems sharing a kernel, then the
- * copy-back).  A call with the same launch shapes as the previous one (same
- * problems and sample counts; seeds may differ) replays that graph. */
+/* Batched scan (no reference counterpart): n problems of uneven size in one
+ * hipGraph.  The problems are bucketed by (kernel path, eps orders, J-table
+ * placement, block size, edge and vertex counts); each bucket is one persistent
+ * launch of a kernel that holds every topology of the bucket as a
+ * compile-time graph spec and whose blocks take (problem, block of samples)
+ * units from a work queue, costliest first.  Every problem gets the samples
+ * and moments of its own ftr_integrate call (up to summation order).  A call
+ * with the same launch shapes as the previous one (same problems and sample
+ * counts; seeds may differ) replays that graph. */
 int ftr_integrate_batch(ftr_ctx* ctx, int n, ftr_problem* const* probs, const uint64_t* N,
                         const uint64_t* seeds, ftr_moments* out);
+/* Compile (no GPU needed) the bucket kernels that ftr_integrate_batch would
+ * use for these n problems into the kernel cache (problems the
+ * preprocessing rejects are skipped). */
+int ftr_precompile_batch(int n, const ftr_input* in, char* err, int errlen);
 /* Number of batch calls served by replaying the previous graph. */
 int ftr_batch_graph_replays(const ftr_ctx* ctx);
 
