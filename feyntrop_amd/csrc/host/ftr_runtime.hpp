@@ -33,7 +33,7 @@ struct Variant {
 
 struct Module {
   hipModule_t mod = nullptr;
-  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr, f_bat = nullptr;
+  hipFunction_t f_int = nullptr, f_rep = nullptr, f_smp = nullptr;
 };
 
 /* concurrent streams of a batch graph's bucket launches */

@@ -374,7 +374,6 @@ static Module load_module(ftr_ctx* c, const std::string& src) {
   HIPCHK(hipModuleGetFunction(&m.f_int, m.mod, "ftrk_int"));
   HIPCHK(hipModuleGetFunction(&m.f_rep, m.mod, "ftrk_rep"));
   HIPCHK(hipModuleGetFunction(&m.f_smp, m.mod, "ftrk_smp"));
-  HIPCHK(hipModuleGetFunction(&m.f_bat, m.mod, "ftrk_bat"));
   c->modules[src] = m;
   return m;
 }

@@ -70,8 +70,8 @@ struct Launch {
   double* partials;        /* nblocks x (2 + 4K) */
 };
 
-/* One problem of a batched launch: its launch data plus the range of global
- * blocks that work on it (items are sorted by block0). */
+/* One problem of a batched launch: its launch data (block0 and nblocks
+ * describe its range of virtual blocks). */
 struct BatchItem {
   Launch a;
   uint32_t block0;

@@ -1388,20 +1388,6 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   }
 }
 
-/* Batched scan: every block belongs to one problem of the item list (found
- * by binary search on block0; uniform per block). */
-template <class G, int K, int MODE, int TMODE>
-__device__ inline void batch_body(const BatchItem* items, int nitems) {
-  int lo = 0, hi = nitems - 1;
-  const uint32_t b = blockIdx.x;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].block0 <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  integrate_body<G, K, MODE, TMODE>(items[lo].a, b - items[lo].block0);
-}
-
 /* Persistent batched launch (no reference counterpart): the next unit of
  * work for this block, or nwork when the list is exhausted.  Thread 0 takes
  * the unit from the counter; the two barriers publish it to the block and
@@ -1475,14 +1461,10 @@ __device__ inline uint32_t persistent_next(uint32_t* counter, uint32_t nwork) {
 
 /* Instantiation macro used by the generated kernel units:
  *   FTR_INSTANTIATE(name, Graph, K, MODE, TMODE, BLOCK)
- * defines extern "C" kernels  name_int, name_bat, name_rep, name_smp. */
+ * defines extern "C" kernels  name_int, name_rep, name_smp. */
 #define FTR_INSTANTIATE(NAME, G, K, MODE, TMODE, BLOCK)                                           \
   extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_int(ftr::Launch a) {                 \
     ftr::integrate_body<G, K, MODE, TMODE>(a, blockIdx.x);                                        \
-  }                                                                                               \
-  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_bat(const ftr::BatchItem* items,      \
-                                                                   int nitems) {                  \
-    ftr::batch_body<G, K, MODE, TMODE>(items, nitems);                                            \
   }                                                                                               \
   extern "C" __global__ void __launch_bounds__(256)                                               \
       NAME##_rep(const ftr::DevParams* prm, uint64_t n, const double* X, const double* px,        \
