@@ -34,7 +34,7 @@
  */
 #pragma once
 
-#ifndef __HIPCC_RTC__
+#if !defined(__HIPCC_RTC__) && !defined(FTR_HOST_BUILD)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #endif
@@ -116,9 +116,9 @@ __host__ __device__ inline cplx clog_(cplx z) {
  * 2^-e, e the exponent of max(|re|, |im|), so that |z 2^-e|^2 lies in
  * [1/4, 2), and log|z| = log(|z 2^-e|^2)/2 + e log 2.  z = 0 gives -inf,
  * infinite or NaN parts give inf or NaN, as clog_ does. */
-__device__ inline cplx clog_fast(cplx z) {
+__host__ __device__ inline cplx clog_fast(cplx z) {
   const double mx = fmax(fabs(z.re), fabs(z.im));
-  const int e = __builtin_amdgcn_frexp_exp(mx);
+  const int e = ftrm::frexp_exp(mx);
   const double zr = __builtin_ldexp(z.re, -e), zi = __builtin_ldexp(z.im, -e);
   double lr = fma(0.5, log_pos(fma(zr, zr, zi * zi)), (double)e * 0x1.62e42fefa39efp-1);
   if (mx == 0.0) lr = -__builtin_inf();
