@@ -130,6 +130,8 @@ int dd_lds_override() {
   return o;
 }
 
+static int p_lowrank(const ftrh::Prep& pr, double* q, double* eta);
+
 /* The graph spec of a problem's kernel (struct `gname`, and `gname`P for the
  * parameters of a kernel specialised to the kinematic point), followed by the
  * kernel instantiations unless `instantiate` is false (bucket kernels list
@@ -175,11 +177,26 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
     darr("nu", d.nu, FTR_MAX_E);
     darr("m2", d.m2, FTR_MAX_E);
     darr("P", d.P, FTR_MAX_V * FTR_MAX_V);
+    darr("Pq", d.Pq, FTR_MAX_V * FTR_MAX_V);
+    darr("Peta", d.Peta, FTR_MAX_V);
     s << "};\n";
   }
   s << "\nstruct " << gname << " {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
   s << "  static constexpr bool DDL = " << (dd_in_lds(v, g.E) ? "true" : "false") << ";\n";
   s << "  static constexpr int BLK = " << v.block << ";\n";
+  /* rank of P's external block (trace and derivatives in low-rank form) */
+  s << "  static constexpr int PR = " << p_lowrank(pr, nullptr, nullptr) << ";\n";
+  {
+    static const bool allow = [] {
+      const char* e = getenv("FEYNTROP_AMD_NU1");   /* =0: always the summed-angle sincos */
+      return !(e && !strcmp(e, "0"));
+    }();
+    bool nu1 = allow;
+    for (int e = 0; e < g.E; e++) nu1 = nu1 && g.nu[e] == 1.0;
+    /* every edge weight 1: the deformation's phase is the product of the
+     * edges' phases (ftr_device.hpp integrand) */
+    s << "  static constexpr bool NU1 = " << (nu1 ? "true" : "false") << ";\n";
+  }
   /* in-step row prefetch of the sampler (ftr_device.hpp sample_point):
    * generic deformed-path kernels */
   s << "  static constexpr bool PF = " << ((v.mode == 2 && !v.cparams) ? "true" : "false") << ";\n";
@@ -306,6 +323,73 @@ std::string full_source(const ftrh::Prep& pr, const Variant& v, bool learned) {
   return src;
 }
 
+/* The vertices with external momentum in elimination order (the order of
+ * the trailing block of the kernels' matrices: ascending vertex number). */
+static std::vector<int> ext_order(const ftrh::Prep& pr) {
+  std::vector<char> isext(pr.g.V, 0);
+  for (int x : pr.ext) isext[x] = 1;
+  std::vector<int> o;
+  for (int u = 0; u < pr.g.V; u++)
+    if (isext[u]) o.push_back(u);
+  return o;
+}
+
+/* P's external block B (symmetric, rows summing to zero by momentum
+ * conservation) as sum_r eta_r q_r q_r^T: cyclic Jacobi eigen-decomposition
+ * in long double, q_r = sqrt|lambda_r| v_r, eta_r = sign lambda_r, with the
+ * eigenvalues below 1e-14 of the largest (the null direction 1 of
+ * conservation) dropped.  q (rows of FTR_MAX_V) and eta may be null. */
+static int p_lowrank(const ftrh::Prep& pr, double* q, double* eta) {
+  const std::vector<int> ex = ext_order(pr);
+  const int n = (int)ex.size();
+  std::vector<long double> B((size_t)n * n), Vm((size_t)n * n, 0.0L);
+  for (int a = 0; a < n; a++)
+    for (int b = 0; b < n; b++) B[(size_t)a * n + b] = pr.P[(size_t)ex[a] * pr.g.V + ex[b]];
+  for (int a = 0; a < n; a++) Vm[(size_t)a * n + a] = 1.0L;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    long double off = 0;
+    for (int a = 0; a < n; a++)
+      for (int b = a + 1; b < n; b++) off += B[(size_t)a * n + b] * B[(size_t)a * n + b];
+    if (off < 1e-60L) break;
+    for (int a = 0; a < n; a++)
+      for (int b = a + 1; b < n; b++) {
+        const long double apq = B[(size_t)a * n + b];
+        if (apq == 0) continue;
+        const long double th = (B[(size_t)b * n + b] - B[(size_t)a * n + a]) / (2 * apq);
+        const long double t = (th >= 0 ? 1 : -1) / (fabsl(th) + sqrtl(th * th + 1));
+        const long double c = 1 / sqrtl(t * t + 1), s = t * c;
+        for (int k = 0; k < n; k++) { /* B <- J^T B J */
+          const long double bka = B[(size_t)k * n + a], bkb = B[(size_t)k * n + b];
+          B[(size_t)k * n + a] = c * bka - s * bkb;
+          B[(size_t)k * n + b] = s * bka + c * bkb;
+        }
+        for (int k = 0; k < n; k++) {
+          const long double bak = B[(size_t)a * n + k], bbk = B[(size_t)b * n + k];
+          B[(size_t)a * n + k] = c * bak - s * bbk;
+          B[(size_t)b * n + k] = s * bak + c * bbk;
+        }
+        for (int k = 0; k < n; k++) {
+          const long double vka = Vm[(size_t)k * n + a], vkb = Vm[(size_t)k * n + b];
+          Vm[(size_t)k * n + a] = c * vka - s * vkb;
+          Vm[(size_t)k * n + b] = s * vka + c * vkb;
+        }
+      }
+  }
+  long double lmax = 0;
+  for (int a = 0; a < n; a++) lmax = std::max(lmax, fabsl(B[(size_t)a * n + a]));
+  int R = 0;
+  for (int a = 0; a < n; a++) {
+    const long double lam = B[(size_t)a * n + a];
+    if (lmax == 0 || fabsl(lam) <= 1e-14L * lmax) continue;
+    const long double sc = sqrtl(fabsl(lam));
+    if (q)
+      for (int b = 0; b < n; b++) q[(size_t)R * FTR_MAX_V + b] = (double)(sc * Vm[(size_t)b * n + a]);
+    if (eta) eta[R] = lam > 0 ? 1.0 : -1.0;
+    R++;
+  }
+  return R;
+}
+
 ftr::DevParams make_params(const ftrh::Prep& pr) {
   ftr::DevParams d;
   memset(&d, 0, sizeof d);
@@ -326,6 +410,7 @@ ftr::DevParams make_params(const ftrh::Prep& pr) {
   }
   for (int i = 0; i < pr.g.V; i++)
     for (int j = 0; j < pr.g.V; j++) d.P[i * FTR_MAX_V + j] = pr.P[(size_t)i * pr.g.V + j];
+  p_lowrank(pr, d.Pq, d.Peta);
   return d;
 }
 

@@ -43,6 +43,11 @@ struct DevParams {
   double nu[FTR_MAX_E];
   double m2[FTR_MAX_E];
   double P[FTR_MAX_V * FTR_MAX_V];
+  /* P's block on the vertices with external momentum (in elimination
+   * order) as sum_r Peta[r] q_r q_r^T, q_r = Pq[r * FTR_MAX_V + a] over the
+   * external positions a, Peta = +-1; r < the graph spec's PR */
+  double Pq[FTR_MAX_V * FTR_MAX_V];
+  double Peta[FTR_MAX_V];
 };
 
 /* Per-state record of the row-table placement (table mode 3). */

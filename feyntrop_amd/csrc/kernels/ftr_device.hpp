@@ -358,11 +358,11 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
 #define FTR_SYM(A, i, j) (((i) >= (j)) ? A[FTR_TRI(i, j)] : A[FTR_TRI(j, i)])
 
 /* In-place right-looking LDL^T of a packed symmetric matrix (real or
- * complex-symmetric); returns the product of the pivots and their inverses
- * in dinv. */
+ * complex-symmetric); returns prod0 times the product of the pivots, and
+ * their inverses in dinv. */
 template <int V>
-__host__ __device__ inline double ldlt_inplace(double (&M)[V * (V + 1) / 2], double (&dinv)[V]) {
-  double prod = 1.0;
+__host__ __device__ inline double ldlt_inplace(double (&M)[V * (V + 1) / 2], double (&dinv)[V],
+                                               double prod = 1.0) {
 #pragma unroll
   for (int k = 0; k < V; k++) {
     const double d = M[FTR_TRI(k, k)];
@@ -382,8 +382,8 @@ __host__ __device__ inline double ldlt_inplace(double (&M)[V * (V + 1) / 2], dou
 }
 
 template <int V>
-__host__ __device__ inline cplx ldlt_inplace(cplx (&M)[V * (V + 1) / 2], cplx (&dinv)[V]) {
-  cplx prod = cmk(1.0, 0.0);
+__host__ __device__ inline cplx ldlt_inplace(cplx (&M)[V * (V + 1) / 2], cplx (&dinv)[V], double prod0 = 1.0) {
+  cplx prod = cmk(prod0, 0.0);
 #pragma unroll
   for (int k = 0; k < V; k++) {
     const cplx d = M[FTR_TRI(k, k)];
@@ -402,36 +402,37 @@ __host__ __device__ inline cplx ldlt_inplace(cplx (&M)[V * (V + 1) / 2], cplx (&
   return prod;
 }
 
-/* tr(A^-1 P) over the trailing external block from the factors of A (packed
- * L below the diagonal, dinv); P entries at external vertices, the deleted
- * vertex vs excluded.  Only positions >= V - NEXT are touched. */
+/* tr(A^-1 P) from the factors of A (packed L below the diagonal, dinv).  P's
+ * block on the external vertices is sum_r eta_r q_r q_r^T (PR terms,
+ * host-computed), so tr(A^-1 P) = sum_r eta_r q_r^T A^-1 q_r, and with
+ * A = L D L^T, q^T A^-1 q = sum_i y_i^2 / d_i for y = L^-1 q.  y vanishes on
+ * the internal positions (they come first and q is zero there), so each term
+ * is one forward substitution on the trailing external block: no back
+ * substitution, PR <= NEXT - 1 terms instead of NEXT solves.  Complex
+ * symmetric A: the same with y_i^2 (no conjugate).  For A = La + alpha
+ * e_v e_v^T (the Laplacian with v*'s diagonal raised, see integrand) A^-1 is
+ * the reduced inverse plus 1 1^T / alpha, whose trace against P vanishes
+ * (q_r is orthogonal to 1), so this is tr(La_red^-1 P_red). */
 template <class G, class T, class PT>
 __host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], const T (&dinv)[G::V],
-                                         const PT& p, int vs) {
-  constexpr int V = G::V, N0 = G::V - G::NEXT;
+                                         const PT& p) {
+  constexpr int V = G::V, NX = G::NEXT, N0 = G::V - G::NEXT;
   T tr = T{};
 #pragma unroll
-  for (int a = 0; a < G::NEXT; a++) {
-    const int pj = N0 + a;               /* position of the column */
-    const int vj = G::ord[pj];
-    T y[V];
+  for (int r = 0; r < G::PR; r++) {
+    T y[NX > 0 ? NX : 1];
 #pragma unroll
-    for (int i = N0; i < V; i++) y[i] = (i == pj) ? T{1.0} : T{};
+    for (int a = 0; a < NX; a++) y[a] = T{p.Pq[r * FTR_MAX_V + a]};
 #pragma unroll
-    for (int i = N0; i < V; i++)
+    for (int a = 0; a < NX; a++)
 #pragma unroll
-      for (int q = pj; q < i; q++) y[i] = fms(y[i], M[FTR_TRI(i, q)], y[q]);
+      for (int b = 0; b < a; b++) y[a] = fms(y[a], M[FTR_TRI(N0 + a, N0 + b)], y[b]);
+    T s = T{};
 #pragma unroll
-    for (int i = N0; i < V; i++) y[i] = y[i] * dinv[i];
-#pragma unroll
-    for (int i = V - 1; i >= N0; i--)
-#pragma unroll
-      for (int q = i + 1; q < V; q++) y[i] = fms(y[i], M[FTR_TRI(q, i)], y[q]);
-    T colsum = T{};
-#pragma unroll
-    for (int b = 0; b < G::NEXT; b++) colsum = colsum + p.P[G::ord[N0 + b] * FTR_MAX_V + vj] * y[N0 + b];
-    if (vj != vs) tr = tr + colsum;
+    for (int a = 0; a < NX; a++) s = s + (y[a] * y[a]) * dinv[N0 + a];
+    tr = tr + p.Peta[r] * s;
   }
+  (void)V;
   return tr;
 }
 
@@ -481,8 +482,26 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
   double xprod = 1.0;
 #pragma unroll
   for (int e = 0; e < E; e++) xprod *= X[e];
-  /* Laplacian with weights 1/x (laplacian.hpp:14-67) in elimination order,
-   * v* row/col -> identity */
+  /* Laplacian with weights 1/x (laplacian.hpp:14-67) in elimination order.
+   * The reference deletes v*'s row and column; here v*'s diagonal is raised
+   * by alpha instead: for a connected graph's Laplacian La (null vector 1),
+   *   det(La + alpha e_v e_v^T) = alpha det(La_red),
+   *   (La + alpha e_v e_v^T)^-1 = La_red^-1 (zero row/col at v) + 1 1^T / alpha,
+   * and the 1 1^T term drops out of every quantity used below (edge
+   * differences u_e^T G u_f with u_e^T 1 = 0, and contractions with P, whose
+   * rows sum to zero).  alpha = 2^40 sum(1/x), a power of two far above
+   * every entry: v*'s pivot is ~alpha, its couplings reach the other pivots
+   * at 2^-40 relative, so the elimination is numerically the deletion, and
+   * dividing alpha out (the product starts at 1/alpha) is exact.  One select
+   * per diagonal entry replaces one per matrix entry. */
+  double alpha;
+  {
+    double sxi = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; e++) sxi += Xi[e];
+    alpha = __builtin_ldexp(1.0, ftrm::frexp_exp(sxi) + 40);
+  }
+  const double ialpha = 1.0 / alpha;
   double M[NT];
 #pragma unroll
   for (int q = 0; q < NT; q++) M[q] = 0.0;
@@ -494,12 +513,9 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     M[(a > b) ? FTR_TRI(a, b) : FTR_TRI(b, a)] -= Xi[e];
   }
 #pragma unroll
-  for (int i = 0; i < V; i++)
-#pragma unroll
-    for (int j = 0; j <= i; j++)
-      if (G::ord[i] == vs || G::ord[j] == vs) M[FTR_TRI(i, j)] = (i == j) ? 1.0 : 0.0;
+  for (int i = 0; i < V; i++) M[FTR_TRI(i, i)] += (G::ord[i] == vs) ? alpha : 0.0;
   double dinv[V];
-  const double psi = ldlt_inplace<V>(M, dinv) * xprod;
+  const double psi = ldlt_inplace<V>(M, dinv, ialpha) * xprod;
 
   if constexpr (MODE != 2) {
     double R = p.IGtr * pow_mD(p, psi);
@@ -511,7 +527,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
       out[0] = cmk(R, 0.0);
       return true;
     }
-    const double tr = trace_inv_P<G, double, PT>(M, dinv, p, vs);
+    const double tr = trace_inv_P<G, double, PT>(M, dinv, p);
     double dot = 0.0;
 #pragma unroll
     for (int e = 0; e < E; e++) dot += p.m2[e] * X[e];
@@ -530,14 +546,10 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     return true;
   } else {
     /* ---- analytic continuation (feynman_integral.hpp:278-362) ---- */
-    /* Gm = reduced inverse, zero on v*: Gm = L^-T D^-1 L^-1 (positions).
-     * v*'s identity row and column leave L and L^-1 with a zero row and
-     * column there (off the diagonal), so every entry of Gm in v*'s row or
-     * column is 0 except the diagonal, which is v*'s inverse pivot: zeroing
-     * that pivot gives the reduced inverse without a select per entry. */
-    double dz[V];
-#pragma unroll
-    for (int k = 0; k < V; k++) dz[k] = (G::ord[k] == vs) ? 0.0 : dinv[k];
+    /* Gm = (La + alpha e_v e_v^T)^-1 = L^-T D^-1 L^-1 (positions): the
+     * reduced inverse plus 1 1^T / alpha, which neither the edge differences
+     * nor LPGPLT = Gm P Gm (P 1 = 0) see */
+    const double (&dz)[V] = dinv;
     double Gm[NT];
     {
       double Li[NT]; /* L^-1, unit lower (diagonal unused) */
@@ -564,30 +576,28 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
           Gm[FTR_TRI(i, j)] = s;
         }
     }
-    /* LPGPLT = Gm P Gm, only needed through P's external support */
-    double LP[NT];
+    /* LPGPLT = Gm P Gm is needed only through its edge differences
+     * u_a^T LPGPLT u_b (u_a = e_{k_a} - e_{l_a}); with P's external block
+     * sum_r eta_r q_r q_r^T these are sum_r eta_r dq[r][a] dq[r][b],
+     * dq[r][a] = (Gm q_r)[k_a] - (Gm q_r)[l_a]: no V x V product and no
+     * cancellation of large LPGPLT entries in the differences */
+    constexpr int PRK = G::PR > 0 ? G::PR : 1;
+    double dq[PRK][E];
     {
       constexpr int N0 = V - G::NEXT;
-      double Y[V][G::NEXT > 0 ? G::NEXT : 1];
 #pragma unroll
-      for (int u = 0; u < V; u++)
+      for (int r = 0; r < G::PR; r++) {
+        double u[V];
 #pragma unroll
-        for (int b = 0; b < G::NEXT; b++) {
+        for (int w = 0; w < V; w++) {
           double s = 0.0;
 #pragma unroll
-          for (int a = 0; a < G::NEXT; a++)
-            s += FTR_SYM(Gm, u, N0 + a) * p.P[G::ord[N0 + a] * FTR_MAX_V + G::ord[N0 + b]];
-          Y[u][b] = s;
+          for (int a = 0; a < G::NEXT; a++) s += FTR_SYM(Gm, w, N0 + a) * p.Pq[r * FTR_MAX_V + a];
+          u[w] = s;
         }
 #pragma unroll
-      for (int u = 0; u < V; u++)
-#pragma unroll
-        for (int w = 0; w <= u; w++) {
-          double s = 0.0;
-#pragma unroll
-          for (int b = 0; b < G::NEXT; b++) s += Y[u][b] * FTR_SYM(Gm, N0 + b, w);
-          LP[FTR_TRI(u, w)] = s;
-        }
+        for (int a = 0; a < E; a++) dq[r][a] = u[G::pos[G::hi[a]]] - u[G::pos[G::lo[a]]];
+      }
     }
     /* first and second derivatives of pphi (laplacian.hpp:102-215) */
     double th[E];
@@ -598,7 +608,9 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
 #pragma unroll
     for (int a = 0; a < E; a++) {
       const int ka = G::pos[G::hi[a]], la = G::pos[G::lo[a]];
-      const double s = FTR_SYM(LP, ka, ka) + FTR_SYM(LP, la, la) - 2.0 * FTR_SYM(LP, ka, la);
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < G::PR; r++) s += p.Peta[r] * (dq[r][a] * dq[r][a]);
       const double t = FTR_SYM(Gm, ka, ka) + FTR_SYM(Gm, la, la) - 2.0 * FTR_SYM(Gm, ka, la);
       const double xa = Xi[a] * Xi[a];
       const double xas = xa * s;
@@ -607,7 +619,9 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
 #pragma unroll
       for (int b = 0; b < a; b++) {
         const int kb = G::pos[G::hi[b]], lb = G::pos[G::lo[b]];
-        const double s2 = FTR_SYM(LP, ka, kb) + FTR_SYM(LP, la, lb) - FTR_SYM(LP, ka, lb) - FTR_SYM(LP, la, kb);
+        double s2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < G::PR; r++) s2 += p.Peta[r] * (dq[r][a] * dq[r][b]);
         const double t2 = FTR_SYM(Gm, ka, kb) + FTR_SYM(Gm, la, lb) - FTR_SYM(Gm, ka, lb) - FTR_SYM(Gm, la, kb);
         const double xb = Xi[b] * Xi[b];
         dd[FTR_TRI(a, b) * DS] = -2.0 * xb * xa * s2 * t2;
@@ -641,7 +655,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
         }
       }
     }
-    jdet = xprod * jdet;
+    if constexpr (!G::NU1) jdet = xprod * jdet;
     /* deformation: theta = def_ref (d pphi + m^2), z = x e^{i theta} */
     cplx cX[E], cXi[E];
     double xne = 0.0;
@@ -652,10 +666,11 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
       sincos_fast(th[e], &sn, &cs);
       cX[e] = cmk(X[e] * cs, X[e] * sn);
       cXi[e] = cmk(Xi[e] * cs, -Xi[e] * sn);
-      xne += p.nu[e] * th[e];
+      if constexpr (!G::NU1) xne += p.nu[e] * th[e];
     }
-    /* complex Laplacian La(1/z) in elimination order, v* -> identity,
-     * complex-symmetric LDL^T and tr(La(1/z)^-1 P) on the external block */
+    /* complex Laplacian La(1/z) in elimination order with v*'s diagonal
+     * raised by alpha (as for the real one), complex-symmetric LDL^T and
+     * tr(La(1/z)^-1 P) on the external block */
     cplx cdet, ctr;
     {
       cplx CM[NT], cdinv[V];
@@ -670,12 +685,9 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
         CM[o] = CM[o] - cXi[e];
       }
 #pragma unroll
-      for (int i = 0; i < V; i++)
-#pragma unroll
-        for (int j = 0; j <= i; j++)
-          if (G::ord[i] == vs || G::ord[j] == vs) CM[FTR_TRI(i, j)] = cmk((i == j) ? 1.0 : 0.0, 0.0);
-      cdet = ldlt_inplace<V>(CM, cdinv);
-      ctr = trace_inv_P<G, cplx, PT>(CM, cdinv, p, vs);
+      for (int i = 0; i < V; i++) CM[FTR_TRI(i, i)].re += (G::ord[i] == vs) ? alpha : 0.0;
+      cdet = ldlt_inplace<V>(CM, cdinv, ialpha);
+      ctr = trace_inv_P<G, cplx, PT>(CM, cdinv, p);
     }
     cplx cxp = cmk(1.0, 0.0), cdot = cmk(0.0, 0.0);
 #pragma unroll
@@ -686,9 +698,16 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     const cplx cPsi = cxp * cdet;
     const cplx cPXi = cdot - ctr;
     cplx cR = p.IGtr * cpow_mD(p, cPsi);
-    double sn, cs;
-    sincos_fast(xne, &sn, &cs);
-    cR = cR * (jdet * cmk(cs, sn));
+    if constexpr (G::NU1) {
+      /* every nu = 1: exp(i sum nu theta) = prod_e e^{i theta_e} = cxp / prod x,
+       * so (prod x) jdet exp(i sum theta) = jdet cxp (jdet without the
+       * prod x factor here) -- no sincos of the summed angle */
+      cR = cR * (jdet * cxp);
+    } else {
+      double sn, cs;
+      sincos_fast(xne, &sn, &cs);
+      cR = cR * (jdet * cmk(cs, sn));
+    }
     cR = cR * cpow_mW(p, cPXi);
     /* Branch-free tail: the finiteness flag is returned, not branched on
      * (an early return here made the gfx950 backend at -O2/-O3 produce wrong

@@ -20,7 +20,11 @@ static ftr::DevParams params_from() {
   d.def_ref = P::def_ref; d.inv_dlw = P::inv_dlw; d.iD = P::iD; d.iW = P::iW;
   d.use_iD = P::use_iD; d.use_iW = P::use_iW;
   for (int i = 0; i < FTR_MAX_E; i++) { d.nu[i] = P::nu[i]; d.m2[i] = P::m2[i]; }
-  for (int i = 0; i < FTR_MAX_V * FTR_MAX_V; i++) d.P[i] = P::P[i];
+  for (int i = 0; i < FTR_MAX_V * FTR_MAX_V; i++) {
+    d.P[i] = P::P[i];
+    d.Pq[i] = P::Pq[i];
+  }
+  for (int i = 0; i < FTR_MAX_V; i++) d.Peta[i] = P::Peta[i];
   return d;
 }
 

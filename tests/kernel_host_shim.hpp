@@ -14,7 +14,7 @@
 #define __global__
 #define __shared__
 #define __launch_bounds__(...)
-#define __align__(n) alignas(n)
+#define __align__(n)
 struct ftr_host_dim3 {
   unsigned x = 0, y = 0, z = 0;
 };

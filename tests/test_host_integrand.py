@@ -81,5 +81,6 @@ def test_integrand_host_build_sanitized(oracle, oracle_ld, replay_stats, tmp_pat
           f"(reference build {rs['frac_gt_1e-12']:.5f}) max {d.max():.2e}")
     assert np.median(d) <= 3.0 * max(rs["median"], 5e-15)
     assert np.mean(d > 1e-12) <= 2.0 * rs["frac_gt_1e-12"] + 1e-3
+    assert np.mean(e_host > 1e-12) <= np.mean(e_ref > 1e-12) + 2e-4
     for q in (0.5, 0.9, 0.99):
         assert np.quantile(e_host, q) <= 2.0 * np.quantile(e_ref, q) + 2e-15, q
