@@ -32,14 +32,14 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # HBM traffic of one ftrk_int launch (N = 1e8) from two rocprofv3 --pmc runs of
 # this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, timed launches),
 # each scaled by what the same counter read for a 256 MiB device-to-device
-# copy in the same run (FETCH 131086.9 KiB, WRITE 262144 KiB for 262144 KiB
+# copy in the same run (FETCH 131088.25 KiB, WRITE 262144 KiB for 262144 KiB
 # moved).  Per kernel kind: generic (parameters read from memory) and
 # specialised to the kinematic point; both with the row-table placement.
-# Summary and commands under profiles/.
-_FETCH_SCALE = 262144 / 131086.9
+# Summary and commands in profiles/ftrk_int_double_box.md.
+_FETCH_SCALE = 262144 / 131088.25
 PMC_TRAFFIC_BYTES = {
-    ("double_box", "generic"): (419.9 * _FETCH_SCALE + 24.0) * 1024,
-    ("double_box", "specialised"): (358.5 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "generic"): (379.5 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "specialised"): (366.0 * _FETCH_SCALE + 24.0) * 1024,
 }
 
 
@@ -317,6 +317,9 @@ def main():
                     help="after the run, one device-to-device copy of this many MiB (the "
                          "reference kernel for scaling HBM traffic counters)")
     ap.add_argument("--scan-points", type=int, default=4096)
+    ap.add_argument("--learned-only", action="store_true",
+                    help="profiling run: train the learned proposal on the 3-loop ladder and integrate "
+                         "N = 1e8 twice with it (one JSON line of kernel times, not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -325,6 +328,18 @@ def main():
 
     import feyntrop_amd
     from feyntrop_amd import workloads as W
+
+    if args.learned_only:
+        eng = feyntrop_amd.Engine(0)
+        prob = W.triple_ladder()
+        p = eng.prepare(prob)
+        L = feyntrop_amd.Learner(eng, p, init_seed=1)
+        st = L.train(400, 1 << 18, seed=100, lr=3e-3)
+        use, _, _ = L.select(n_probe=4_000_000, seed=7)
+        ms = [L.integrate(int(prob["N"]), 12 + i).kernel_ms for i in range(2)]
+        print(json.dumps({"workload": "learned proposal, triple_ladder", "selected": "learned" if use else "plain",
+                          "train_rel2_last": float(st[-1, 0]), "kernel_ms": ms}), flush=True)
+        return
 
     prob = (W.all_problems().get(args.workload) or W.LARGE[args.workload])()
     if args.N:
