@@ -441,10 +441,10 @@ class Learner:
     def statistics(self, samples, seed):
         """One training step's record and forward-KL statistics, without the
         update: dict of c (N), ly ((E-1) x N), path (N), gC (2^E x E), gS
-        (2^E), gsum (2) and dY (rows x 32)."""
+        (2^E x E), gsum (2) and dY (rows x 32)."""
         n, E, N = 1 << self.E, self.E, int(samples)
         out = dict(c=np.zeros(N), ly=np.zeros((max(E - 1, 1), N)), path=np.zeros(N, np.uint64),
-                   gC=np.zeros((n, E)), gS=np.zeros(n), gsum=np.zeros(2), dY=np.zeros((self.rows, 32), np.float32))
+                   gC=np.zeros((n, E)), gS=np.zeros((n, E)), gsum=np.zeros(2), dY=np.zeros((self.rows, 32), np.float32))
         self._chk(lib().ftr_learner_statistics(self.engine.h, self.h, N, int(seed), out["c"].ctypes.data,
                                                out["ly"].ctypes.data, out["path"].ctypes.data,
                                                out["gC"].ctypes.data, out["gS"].ctypes.data,
@@ -453,8 +453,8 @@ class Learner:
 
     def tables(self):
         n, E = 1 << self.E, self.E
-        lw, lratio = np.zeros((n, E)), np.zeros((n, E))
-        lZ, lia, lrad, ldiff = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        lw, lratio, lia, lrad, ldiff = (np.zeros((n, E)) for _ in range(5))
+        lZ = np.zeros(n)
         self._chk(lib().ftr_learner_tables(self.engine.h, self.h, lw.ctypes.data, lZ.ctypes.data,
                                            lratio.ctypes.data, lia.ctypes.data, lrad.ctypes.data,
                                            ldiff.ctypes.data))

@@ -34,7 +34,7 @@ static_assert(NPAR == FTR_LEARN_NPARAMS, "parameter layout");
 constexpr int MAX_E = 16;          /* cut order packed 4 bits per step */
 constexpr double DMAX = 8.0;       /* |Delta| bound (cut-weight tilt) */
 constexpr int HIST_E = 10;         /* histogram in LDS up to this E (ftr_device.hpp) */
-constexpr size_t hist_len(int E) { return ((size_t)E + 1) * ((size_t)1 << E) + 2; }
+constexpr size_t hist_len(int E) { return (size_t)E * ((size_t)1 << E) + 2; } /* = ftr::learn_hist_len */
 constexpr int LEARN_BLOCK = 256;  /* one wave per SIMD per block: the full register budget per lane */
 constexpr double SMAX = 0.6;       /* |sigma| bound: a = r e^sigma < 2r keeps p/q of finite variance */
 constexpr int MAX_SPLITS = 64;     /* split-K blocks of a weight-gradient GEMM */
@@ -97,7 +97,7 @@ struct ftr_learner {
   double *d_lw = nullptr, *d_lZ = nullptr, *d_lratio = nullptr, *d_lia = nullptr, *d_lrad = nullptr,
          *d_ldiff = nullptr, *d_r = nullptr, *d_gC = nullptr, *d_gS = nullptr, *d_gsum = nullptr;
   ftr::LQuad* d_lst = nullptr; /* per-state records read by the sampler */
-  double* d_ltw = nullptr;      /* per-cut ratio factors w(g\e)/lw(g,e) */
+  ftr::LEdge* d_led = nullptr;  /* per-cut records read by the sampler */
   int adam_t = 0;
   bool tables_valid = false;
   bool use_plain = false; /* set by ftr_learner_select when the plain sampler wins */
@@ -221,7 +221,7 @@ void build_tables(ftr_ctx* c, ftr_learner* l) {
   double dmax = DMAX, smax = SMAX;
   void* args[] = {&l->d_Y,    &E,          &l->p->d_w, &l->p->d_jr, &l->d_r,    &l->d_lw,          &l->d_lZ,
                   &l->d_lratio, &l->d_lia, &l->d_lrad, &l->d_ldiff, &dmax, &smax, &l->d_lst, &l->p->d_mml,
-                  &l->d_ltw};
+                  &l->d_led};
   launch(l->mlp.tables, (l->NT + 255) / 256, 1, 256, c->stream, args);
 }
 
@@ -243,7 +243,7 @@ ftr::LearnLaunch learn_launch(ftr_learner* l, const Geometry& G, uint64_t N) {
   L.a.S = G.S;
   L.a.partials = G.d_partials;
   L.lst = l->d_lst;
-  L.ltw = l->d_ltw;
+  L.led = l->d_led;
   L.lw = l->d_lw;
   L.lZ = l->d_lZ;
   L.lratio = l->d_lratio;
@@ -316,7 +316,9 @@ void ensure_records(ftr_learner* l, uint64_t samples) {
 Geometry step_statistics(ftr_ctx* c, ftr_learner* l, uint64_t samples, uint64_t seed) {
   ftr_problem* p = l->p;
   const int E = l->E, NQ = 2 + 4 * p->prep.K;
-  const size_t acc_lds = (size_t)(2 + 2 * E + E * E + E * (E - 1) / 2 * (1 + E)) * 8;
+  /* = ftr::learn_acc_lds: the sums and the C and S statistics of the three
+   * top levels, E + E^2 + E (E-1)/2 E entries each */
+  const size_t acc_lds = (size_t)(2 + 2 * (E + E * E + E * (E - 1) / 2 * E)) * 8;
   forward(c, l);
   build_tables(c, l);
   /* fixed-point statistics: the scale's maximum and (E > HIST_E) the one
@@ -374,7 +376,10 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->mlp = load_mlp(c);
     l->var = p->var;
     l->var.block = LEARN_BLOCK;
-    l->var.cparams = 0; /* the learned kernels read the parameters from memory */
+    /* the learned kernels are specialised to the problem's kinematic point
+     * (its parameters compile-time constants, as ftr_set_specialized does
+     * for the plain kernel): a learner belongs to one point anyway */
+    l->var.cparams = 1;
     l->lds = variant_lds(l->var, E);
     const hipModule_t gm = load_module_raw(c, full_source(p->prep, l->var, true));
     HIPCHK(hipModuleGetFunction(&l->f_lev, gm, "ftrl_lev"));
@@ -421,14 +426,14 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->d_lratio = dalloc<double>(l, NT * E);
     l->d_gC = dalloc<double>(l, NT * E);
     l->d_lZ = dalloc<double>(l, NT);
-    l->d_lia = dalloc<double>(l, NT);
-    l->d_lrad = dalloc<double>(l, NT);
-    l->d_ldiff = dalloc<double>(l, NT);
+    l->d_lia = dalloc<double>(l, NT * E);
+    l->d_lrad = dalloc<double>(l, NT * E);
+    l->d_ldiff = dalloc<double>(l, NT * E);
     l->d_r = dalloc<double>(l, NT);
-    l->d_gS = dalloc<double>(l, NT);
+    l->d_gS = dalloc<double>(l, NT * E);
     l->d_gsum = dalloc<double>(l, 2);
     l->d_lst = dalloc<ftr::LQuad>(l, NT);
-    l->d_ltw = dalloc<double>(l, NT * E);
+    l->d_led = dalloc<ftr::LEdge>(l, NT * E);
     /* initial parameters: hidden layers uniform(+-sqrt(3/fan_in)), zero
      * biases, zero output layer (q = p_trop) */
     std::vector<float> par(NPAR, 0.0f);
@@ -599,7 +604,7 @@ int ftr_learner_statistics(ftr_ctx* c, ftr_learner* l, uint64_t samples, uint64_
     if (ly && E > 1) HIPCHK(hipMemcpyDtoHAsync(ly, l->d_gbuf + N, (E - 1) * N * 8, c->stream));
     if (path) HIPCHK(hipMemcpyDtoHAsync(path, l->d_gpath, N * 8, c->stream));
     if (gC) HIPCHK(hipMemcpyDtoHAsync(gC, l->d_gC, NT * E * 8, c->stream));
-    if (gS) HIPCHK(hipMemcpyDtoHAsync(gS, l->d_gS, NT * 8, c->stream));
+    if (gS) HIPCHK(hipMemcpyDtoHAsync(gS, l->d_gS, NT * E * 8, c->stream));
     if (gsum) HIPCHK(hipMemcpyDtoHAsync(gsum, l->d_gsum, 2 * 8, c->stream));
     if (dY) HIPCHK(hipMemcpyDtoHAsync(dY, l->d_dY, (size_t)l->M * NOUT * 4, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -619,9 +624,9 @@ int ftr_learner_tables(ftr_ctx* c, ftr_learner* l, double* lw, double* lZ, doubl
     if (lw) HIPCHK(hipMemcpyDtoHAsync(lw, l->d_lw, NT * E * 8, c->stream));
     if (lratio) HIPCHK(hipMemcpyDtoHAsync(lratio, l->d_lratio, NT * E * 8, c->stream));
     if (lZ) HIPCHK(hipMemcpyDtoHAsync(lZ, l->d_lZ, NT * 8, c->stream));
-    if (lia) HIPCHK(hipMemcpyDtoHAsync(lia, l->d_lia, NT * 8, c->stream));
-    if (lrad) HIPCHK(hipMemcpyDtoHAsync(lrad, l->d_lrad, NT * 8, c->stream));
-    if (ldiff) HIPCHK(hipMemcpyDtoHAsync(ldiff, l->d_ldiff, NT * 8, c->stream));
+    if (lia) HIPCHK(hipMemcpyDtoHAsync(lia, l->d_lia, NT * E * 8, c->stream));
+    if (lrad) HIPCHK(hipMemcpyDtoHAsync(lrad, l->d_lrad, NT * E * 8, c->stream));
+    if (ldiff) HIPCHK(hipMemcpyDtoHAsync(ldiff, l->d_ldiff, NT * E * 8, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   } catch (std::exception& e) {
     return fail_exc(c, e);
@@ -735,6 +740,7 @@ int ftr_precompile_learned(const ftr_input* in, char* err, int errlen) {
     for (size_t lim : {(size_t)64 * 1024, (size_t)160 * 1024}) {
       Variant v = choose_variant(pr, lim);
       v.block = LEARN_BLOCK;
+      v.cparams = 1;
       code_object(full_source(pr, v, true), nullptr);
     }
   } catch (std::exception& e) {

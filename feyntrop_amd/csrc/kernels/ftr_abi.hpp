@@ -94,24 +94,33 @@ struct BatchWork {
   uint32_t pad;
 };
 
-/* Learned proposal tables (2^E states g; E <= 16):
+/* Learned proposal tables (2^E states g; E <= 16; [g*E+e] entries are zero
+ * for e not in g):
  *   lw[g*E+e]     cut weight of e from g: w(g\e) exp(Delta_e(g))
  *   lZ[g]         sum of lw[g*E+e] over e in g, ascending e
  *   lratio[g*E+e] p_trop(e|g) / q(e|g) = lZ[g] w(g\e) / (J(g) lw[g*E+e])
- *   lia[g]        1 / a(g), a(g) = r(g) exp(sigma(g)): radial exponent after
- *                 arriving at g (plain sampling: a = r)
- *   lrad[g]       log(r/a) = -sigma(g);   ldiff[g]  r(g) - a(g)
+ *   lia[g*E+e]    1 / a(g,e), a(g,e) = r(g\e) exp(sigma(g,e)): exponent of
+ *                 the radius drawn after cutting e from g (plain sampling:
+ *                 a = r(g\e)); zero when g\e is empty (no radius follows)
+ *   lrad[g*E+e]   log(r/a) = -sigma(g,e);   ldiff[g*E+e]  r(g\e) - a(g,e)
  * Training statistics (doubles, sums over a step's samples, accumulated in
- * fixed point): gC[g*E+e] = sum of c over visits of g that cut e; gS[g] =
- * sum of c (1 + a(g) log Y) over the radii drawn after arriving at g;
- * gsum = (sum of c, number of samples), with c = weight * |R_0|. */
-/* Per-state record of the learned tables, one 64-byte load per visited state:
- * lZ(g), A(g) = lZ(g)/J(g), the radial parameters lia, lrad, ldiff and the
- * state's mm | L<<1 flags. */
-struct alignas(64) LQuad {
-  double lZ, A, lia, lrad, ldiff;
+ * fixed point): gC[g*E+e] = sum of c over visits of g that cut e; gS[g*E+e]
+ * = sum of c (1 + a(g,e) log Y) over the radii drawn after cutting e from g;
+ * gsum = (sum of c, number of samples), with c = weight * |R_0|.  The
+ * fixed-point accumulator packs both into 2^E x E slots: C(g,e) at g*E+e
+ * (e in g) and S(g,e) at (g\e)*E+e (a slot whose edge is not in its state,
+ * unused by C), then the two sums. */
+/* Per-state record of the learned tables, one 32-byte load per visited
+ * state: lZ(g), A(g) = lZ(g)/J(g) and the state's mm | L<<1 flags. */
+struct alignas(32) LQuad {
+  double lZ, A;
   uint32_t mmL, pad0;
-  double pad1, pad2;
+  double pad1;
+};
+/* Per-cut record, one 32-byte load per cut: the ratio factor
+ * w(g\e)/lw(g,e) and the radial parameters lia, lrad, ldiff of (g,e). */
+struct alignas(32) LEdge {
+  double tw, lia, lrad, ldiff;
 };
 
 /* Fixed-point accumulator of the training statistics (two's complement). */
@@ -120,7 +129,7 @@ typedef unsigned long long fixacc;
 struct LearnLaunch {
   Launch a;
   const LQuad* lst;
-  const double* ltw;   /* 2^E x E: w(g\e) / lw(g, e), the per-cut ratio factor */
+  const LEdge* led;    /* 2^E x E per-cut records */
   const double* lw;
   const double* lZ;
   const double* lratio;

@@ -1019,7 +1019,9 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
   for (int e = 0; e < E; e++) X[e] = 0.0;
   /* the state's learned weights, all in flight together; the row of the
    * state a cut reaches is requested in the cut's own step, after its
-   * record, so that it is in flight during the radial draw */
+   * record, so that it is in flight during the radial draw (measured: 21.9
+   * ms per 1e8 on the triple ladder against 24.1 ms when each step requests
+   * its own row, double box 12.4 against 12.9 ms) */
   double rv[E];
 #pragma unroll
   for (int q = 0; q < E; q++) rv[q] = L.lw[(size_t)g * E + q];
@@ -1036,7 +1038,8 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     const int e = (cnt < E) ? cnt : 31 - __builtin_clz(g);
     const uint32_t c = g ^ (1u << e);
     ratio *= fpend;
-    fpend = st.A * L.ltw[(size_t)g * E + e];
+    const LEdge ed = L.led[(size_t)g * E + e];
+    fpend = st.A * ed.tw;
     st = L.lst[c];
     if (j < E - 1) {
 #pragma unroll
@@ -1059,10 +1062,10 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     path |= (uint64_t)e << (4 * j);
     if (j < E - 1) {
       const double U = rng.uniform();
-      const double lY = st.lia * log_pos(1. - U);
+      const double lY = ed.lia * log_pos(1. - U);
       lX += lY;
       XX = exp(lX);
-      lrad += st.lrad + st.ldiff * lY;
+      lrad += ed.lrad + ed.ldiff * lY;
       if (KEEP_LY) ly_out[(size_t)j * ly_stride] = lY;
     }
     g = c;
@@ -1087,7 +1090,9 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   constexpr int E = G::E;
   constexpr int NQ = 2 + 4 * K;
   const Launch& a = L.a;
-  const DevParams& p = *a.prm;
+  using PT = typename G::PRM;
+  const ParamsFor<PT> ps{};
+  const PT& p = ps.get(a);
   /* the learned tables are read from L2/HBM: the LDS holds the X slots
    * (TMODE 3) and the reduction */
   double* red = red_area<G, TMODE>(lds);
@@ -1106,7 +1111,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
     learn_point<G, GRAD>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
     if (GRAD) L.gpath[i] = path;
     cplx out[K];
-    if (!integrand<G, K, MODE, DevParams, G::DDL>(p, X, log_scale(p, lF, lGv), out, dds)) {
+    if (!integrand<G, K, MODE, PT, G::DDL>(p, X, log_scale(p, lF, lGv), out, dds)) {
       acc[1] += 1.0;
       if (GRAD) L.gbuf[i] = 0.0;
       continue;
@@ -1141,7 +1146,8 @@ __device__ inline void learn_body(const LearnLaunch& L) {
 
 /* Forward-KL statistics of the recorded samples:
  *   C(g, e) = sum of c over visits of state g that cut e,
- *   S(g)    = sum of c (1 + a(g) lY) over the radii drawn on arriving at g,
+ *   S(g, e) = sum of c (1 + a(g,e) lY) over the radii drawn after cutting
+ *             e from g,
  *   gsum    = (sum of c, number of samples).
  * They are kept in fixed point: every value is rounded to an integer
  * multiple of 2^-k and added in 64-bit integers, so the sums do not depend
@@ -1150,15 +1156,15 @@ __device__ inline void learn_body(const LearnLaunch& L) {
  * samples (learn_max_body) so that N M 2^k < 2^62: no sum can overflow.  The
  * rounding is relative to the largest value (2^-62 N M at most); the
  * gradients are formed in FP32.  Non-finite values count as 0.
- * Layout of an accumulator (learn_hist_len entries): C at g E + e, S at
- * 2^E E + g, then the two sums. */
+ * Layout of an accumulator (learn_hist_len entries): C(g,e) at g E + e
+ * (e in g), S(g,e) at (g\e) E + e (e not in the slot's state, so no C
+ * lives there), then the two sums. */
 __host__ __device__ constexpr int learn_pairs(int E) { return E * (E - 1) / 2; }
-__host__ __device__ constexpr size_t learn_acc_lds(int E) {
-  return (size_t)(2 + 2 * E + E * E + learn_pairs(E) * (1 + E)) * 8;
-}
-__host__ __device__ constexpr size_t learn_hist_len(int E) {
-  return ((size_t)E + 1) * ((size_t)1 << E) + 2;
-}
+/* the cut (C) and radial (S) statistics of the three top levels, each
+ * indexed by (state, cut edge): E + E^2 + pairs * E per kind */
+__host__ __device__ constexpr int learn_top_len(int E) { return E + E * E + learn_pairs(E) * E; }
+__host__ __device__ constexpr size_t learn_acc_lds(int E) { return (size_t)(2 + 2 * learn_top_len(E)) * 8; }
+__host__ __device__ constexpr size_t learn_hist_len(int E) { return (size_t)E * ((size_t)1 << E) + 2; }
 __device__ inline int pair_index(int a, int b) {
   const int hi = a > b ? a : b, lo = a > b ? b : a;
   return hi * (hi - 1) / 2 + lo;
@@ -1169,10 +1175,13 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-/* c (1 + a(cg) lY_j): the radial statistic of step j of sample i */
-__device__ inline double learn_radial(const LearnLaunch& L, double c, uint64_t N, uint64_t i, int j, uint32_t cg) {
-  return c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[cg]);
+/* c (1 + a(g,e) lY_j): the radial statistic of step j of sample i, whose
+ * cut was e from g (k = g E + e) */
+__device__ inline double learn_radial(const LearnLaunch& L, double c, uint64_t N, uint64_t i, int j, size_t k) {
+  return c * (1.0 + L.gbuf[(size_t)(1 + j) * N + i] / L.lia[k]);
 }
+/* accumulator slot of S(g,e): (g\e) E + e, never a slot of C */
+__device__ inline size_t learn_s_slot(uint32_t g, int e, int E) { return (size_t)(g ^ (1u << e)) * E + e; }
 
 __device__ inline bool finite_d(double v) { return __builtin_isfinite(v); }
 
@@ -1206,10 +1215,10 @@ __device__ inline void learn_max_body(const LearnLaunch& L, uint64_t N) {
     const uint64_t path = L.gpath[i];
     uint32_t g = (1u << E) - 1u;
     for (int j = 0; j < E - 1; j++) {
-      const uint32_t cg = g ^ (1u << (int)((path >> (4 * j)) & 15u));
-      const double sv = learn_radial(L, c, N, i, j, cg);
+      const int e = (int)((path >> (4 * j)) & 15u);
+      const double sv = learn_radial(L, c, N, i, j, (size_t)g * E + e);
       if (finite_d(sv)) m = fmax(m, fabs(sv));
-      g = cg;
+      g ^= 1u << e;
     }
   }
   m = wave_max(m);
@@ -1222,26 +1231,22 @@ __device__ inline void learn_max_body(const LearnLaunch& L, uint64_t N) {
 }
 
 /* E > FTR_LEARN_LDS_E: one thread per sample, adding into one accumulator
- * `acc` (zeroed beforehand).  The states of the first three levels (full
- * graph, one and two edges removed) are shared by many samples: their sums
- * go through LDS (fixed indices: removed edge, removed pair) and reach
- * global memory once per block. */
+ * `acc` (zeroed beforehand; slots as learn_hist_len).  The (state, cut)
+ * pairs of the first three levels (full graph, one and two edges removed)
+ * are shared by many samples: their C and S sums go through LDS (fixed
+ * indices: removed edges, then the cut) and reach global memory once per
+ * block. */
 template <int E>
 __device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N, fixacc* acc) {
   extern __shared__ __align__(16) char lds[];
   fixacc* sh = reinterpret_cast<fixacc*>(lds);
-  constexpr int NP = learn_pairs(E);
   constexpr uint32_t NT = 1u << E;
-  fixacc* s_sum = sh;                  /* 2 */
-  fixacc* s_c0 = s_sum + 2;            /* E: cut e0 from the full graph */
-  fixacc* s_s1 = s_c0 + E;             /* E: radius at full \ e0 */
-  fixacc* s_c1 = s_s1 + E;             /* E x E: cut e1 from full \ e0 */
-  fixacc* s_s2 = s_c1 + E * E;         /* NP: radius at full \ {e0, e1} */
-  fixacc* s_c2 = s_s2 + NP;            /* NP x E: cut e2 from full \ {e0, e1} */
-  constexpr int NSH = 2 + 2 * E + E * E + NP * (1 + E);
-  fixacc* aC = acc;
-  fixacc* aS = acc + (size_t)NT * E;
-  fixacc* asum = aS + NT;
+  constexpr int NB = learn_top_len(E);
+  fixacc* s_sum = sh;          /* 2 */
+  fixacc* s_c = s_sum + 2;     /* NB: level 0 (e0), 1 (e0 E + e1), 2 (pair(e0,e1) E + e2) */
+  fixacc* s_s = s_c + NB;      /* NB: the same indices for S */
+  constexpr int NSH = 2 + 2 * NB;
+  fixacc* asum = acc + (size_t)NT * E;
   for (int q = threadIdx.x; q < NSH; q += blockDim.x) sh[q] = 0;
   __syncthreads();
   const double scale = learn_fix_scale(L, N);
@@ -1257,26 +1262,29 @@ __device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N, fixacc* 
       int e0 = 0, e1 = 0;
       for (int j = 0; j < E; j++) {
         const int e = (int)((path >> (4 * j)) & 15u);
-        const uint32_t cg = g ^ (1u << e);
         fixacc qs = 0;
         if (j < E - 1) {
-          const double sv = learn_radial(L, c, N, i, j, cg);
+          const double sv = learn_radial(L, c, N, i, j, (size_t)g * E + e);
           qs = finite_d(sv) ? to_fix(sv, scale) : (fixacc)0;
         }
+        int t = -1;
         if (j == 0) {
           e0 = e;
-          atomicAdd(&s_c0[e], qc);
-          atomicAdd(&s_s1[e], qs);
+          t = e;
         } else if (j == 1) {
           e1 = e;
-          atomicAdd(&s_c1[e0 * E + e], qc);
-          atomicAdd(&s_s2[pair_index(e0, e1)], qs);
-        } else {
-          if (j == 2) atomicAdd(&s_c2[pair_index(e0, e1) * E + e], qc);
-          else atomicAdd(&aC[(size_t)g * E + e], qc);
-          if (j < E - 1) atomicAdd(&aS[cg], qs);
+          t = E + e0 * E + e;
+        } else if (j == 2) {
+          t = E + E * E + pair_index(e0, e1) * E + e;
         }
-        g = cg;
+        if (t >= 0) {
+          atomicAdd(&s_c[t], qc);
+          atomicAdd(&s_s[t], qs);
+        } else {
+          atomicAdd(&acc[(size_t)g * E + e], qc);
+          if (j < E - 1) atomicAdd(&acc[learn_s_slot(g, e, E)], qs);
+        }
+        g ^= 1u << e;
       }
     }
   }
@@ -1287,29 +1295,29 @@ __device__ inline void learn_acc_body(const LearnLaunch& L, uint64_t N, fixacc* 
     if (v == 0) continue;
     if (q < 2) {
       atomicAdd(&asum[q], v);
-    } else if (q < 2 + E) {
-      atomicAdd(&aC[(size_t)full * E + (q - 2)], v);
-    } else if (q < 2 + 2 * E) {
-      atomicAdd(&aS[full ^ (1u << (q - 2 - E))], v);
-    } else if (q < 2 + 2 * E + E * E) {
-      const int k = q - 2 - 2 * E, e0 = k / E, e = k % E;
-      atomicAdd(&aC[(size_t)(full ^ (1u << e0)) * E + e], v);
+      continue;
+    }
+    const bool is_s = q >= 2 + NB;
+    int k = q - 2 - (is_s ? NB : 0);
+    uint32_t g;
+    int e;
+    if (k < E) {
+      g = full;
+      e = k;
+    } else if (k < E + E * E) {
+      k -= E;
+      g = full ^ (1u << (k / E));
+      e = k % E;
     } else {
-      const int k = q - 2 - 2 * E - E * E;
-      int pidx, e = -1;
-      if (k < NP) {
-        pidx = k;
-      } else {
-        pidx = (k - NP) / E;
-        e = (k - NP) % E;
-      }
+      k -= E + E * E;
+      const int pidx = k / E;
+      e = k % E;
       int hi = 1;
       while ((hi + 1) * hi / 2 <= pidx) hi++;
       const int lo = pidx - hi * (hi - 1) / 2;
-      const uint32_t gg = full ^ (1u << hi) ^ (1u << lo);
-      if (e < 0) atomicAdd(&aS[gg], v);
-      else atomicAdd(&aC[(size_t)gg * E + e], v);
+      g = full ^ (1u << hi) ^ (1u << lo);
     }
+    atomicAdd(&acc[is_s ? learn_s_slot(g, e, E) : (size_t)g * E + e], v);
   }
 }
 
@@ -1325,9 +1333,8 @@ __device__ inline void learn_hist_body(const LearnLaunch& L, uint64_t N, fixacc*
     fixacc* h = reinterpret_cast<fixacc*>(lds);
     constexpr uint32_t NT = 1u << E;
     constexpr size_t HL = learn_hist_len(E);
-    fixacc* hC = h;                  /* NT x E */
-    fixacc* hS = h + (size_t)NT * E; /* NT */
-    fixacc* hsum = hS + NT;          /* 2 */
+    fixacc* hCS = h;                  /* NT x E: C and S slots */
+    fixacc* hsum = h + (size_t)NT * E; /* 2 */
     for (size_t q = threadIdx.x; q < HL; q += blockDim.x) h[q] = 0;
     __syncthreads();
     const double scale = learn_fix_scale(L, N);
@@ -1343,13 +1350,12 @@ __device__ inline void learn_hist_body(const LearnLaunch& L, uint64_t N, fixacc*
       uint32_t g = NT - 1u;
       for (int j = 0; j < E; j++) {
         const int e = (int)((path >> (4 * j)) & 15u);
-        const uint32_t cg = g ^ (1u << e);
-        atomicAdd(&hC[(size_t)g * E + e], qc);
+        atomicAdd(&hCS[(size_t)g * E + e], qc);
         if (j < E - 1) {
-          const double sv = learn_radial(L, c, N, i, j, cg);
-          if (finite_d(sv)) atomicAdd(&hS[cg], to_fix(sv, scale));
+          const double sv = learn_radial(L, c, N, i, j, (size_t)g * E + e);
+          if (finite_d(sv)) atomicAdd(&hCS[learn_s_slot(g, e, E)], to_fix(sv, scale));
         }
-        g = cg;
+        g ^= 1u << e;
       }
     }
     atomicAdd(&hsum[0], c_sum);
@@ -1380,9 +1386,19 @@ __device__ inline void learn_reduce_body(const LearnLaunch& L, const fixacc* par
     return;
   }
   const double v = (double)(long long)s / learn_fix_scale(L, N);
-  if (q < (size_t)NT * E) L.gC[q] = v;
-  else if (q < (size_t)NT * E + NT) L.gS[q - (size_t)NT * E] = v;
-  else L.gsum[0] = v;
+  if (q < (size_t)NT * E) {
+    const uint32_t g = (uint32_t)(q / E);
+    const int e = (int)(q % E);
+    if ((g >> e) & 1u) {
+      L.gC[q] = v; /* gS[q] is written by the thread of slot (g\e) E + e */
+    } else {
+      L.gC[q] = 0.0;
+      L.gS[q] = 0.0;
+      L.gS[(size_t)(g | (1u << e)) * E + e] = v;
+    }
+  } else {
+    L.gsum[0] = v;
+  }
 }
 
 /* Learned sampler only (parity): X, the weight p/q and the cut order of

@@ -184,12 +184,13 @@ extern "C" __global__ void mlp_to_bf16(const float* src, unsigned short* dst, ui
 }
 
 /* Learned tables from the network outputs Y (rows = states g, 32 columns:
- * Delta_e in columns 0..E-1, sigma in column E), squashed to
- * Delta = dmax tanh(y/dmax), sigma = smax tanh(y/smax). */
+ * Delta_e(g) in columns 0..E-1, sigma(g,e) of the radius drawn after
+ * cutting e from g in column E+e), squashed to Delta = dmax tanh(y/dmax),
+ * sigma = smax tanh(y/smax). */
 extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double* w, const double* jr,
                                             const double* r, double* lw, double* lZ, double* lratio, double* lia,
                                             double* lrad, double* ldiff, double dmax, double smax, double* lst,
-                                            const uint8_t* mml, double* ltw) {
+                                            const uint8_t* mml, double* led) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (1u << E)) return;
   const float* y = Y + (size_t)g * 32;
@@ -202,27 +203,38 @@ extern "C" __global__ void mlp_build_tables(const float* Y, int E, const double*
   }
   lZ[g] = Z;
   for (int e = 0; e < E; e++) {
-    const double v = lw[(size_t)g * E + e];
+    const size_t k = (size_t)g * E + e;
+    const double v = lw[k];
+    const uint32_t c = g ^ (1u << e);
     const bool in = (g >> e) & 1u;
-    lratio[(size_t)g * E + e] = in ? (Z * w[g ^ (1u << e)]) / (jr[g] * v) : 0.0;
-    ltw[(size_t)g * E + e] = in ? w[g ^ (1u << e)] / v : 0.0;
+    lratio[k] = in ? (Z * w[c]) / (jr[g] * v) : 0.0;
+    /* the radius drawn after cutting e from g (none after the last cut) */
+    double ia = 0.0, sr = 0.0, df = 0.0;
+    if (in && c != 0u) {
+      const double sg = smax * tanh((double)y[E + e] / smax);
+      const double aa = r[c] * exp(sg);
+      ia = 1.0 / aa;
+      sr = -sg;
+      df = r[c] - aa;
+    }
+    lia[k] = ia;
+    lrad[k] = sr;
+    ldiff[k] = df;
+    /* the sampler's per-cut record (ftr_abi.hpp LEdge) */
+    double* d = led + k * 4;
+    d[0] = in ? w[c] / v : 0.0;
+    d[1] = ia;
+    d[2] = sr;
+    d[3] = df;
   }
-  const double sg = smax * tanh((double)y[E] / smax);
-  const double aa = r[g] * exp(sg);
-  lia[g] = 1.0 / aa;
-  lrad[g] = -sg;
-  ldiff[g] = r[g] - aa;
   /* the sampler's per-state record (ftr_abi.hpp LQuad) */
-  double* q = lst + (size_t)g * 8;
+  double* q = lst + (size_t)g * 4;
   q[0] = Z;
   q[1] = Z / jr[g];
-  q[2] = 1.0 / aa;
-  q[3] = -sg;
-  q[4] = r[g] - aa;
-  uint32_t* qi = reinterpret_cast<uint32_t*>(q + 5);
+  uint32_t* qi = reinterpret_cast<uint32_t*>(q + 2);
   qi[0] = mml[g];
   qi[1] = 0;
-  q[6] = q[7] = 0.0;
+  q[3] = 0.0;
 }
 
 /* dL/dY of the forward KL  L = -sum_paths c log q(path) / sum c  from the
@@ -243,6 +255,9 @@ extern "C" __global__ void mlp_table_grad(const float* Y, int E, const double* l
     const double t = tanh((double)y[e] / dmax);
     d[e] = (float)(-(gC[(size_t)g * E + e] - q * visits) / S * (1.0 - t * t));
   }
-  const double t = tanh((double)y[E] / smax);
-  d[E] = (float)(-gS[g] / S * (1.0 - t * t));
+  for (int e = 0; e < E; e++) {
+    if (!((g >> e) & 1u) || g == (1u << e)) continue;
+    const double t = tanh((double)y[E + e] / smax);
+    d[E + e] = (float)(-gS[(size_t)g * E + e] / S * (1.0 - t * t));
+  }
 }

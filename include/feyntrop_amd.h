@@ -248,8 +248,8 @@ int ftr_learner_train(ftr_ctx* ctx, ftr_learner* l, int steps, uint64_t samples,
  *                 0 for a dropped sample)
  *   ly[(E-1) N]   the radial log-draws, ly[j N + i] for step j of sample i
  *   path[N]       the cut order, 4 bits per step
- *   gC[2^E E], gS[2^E], gsum[2]   the forward-KL statistics (ftr_abi.hpp
- *                 LearnLaunch), summed in fixed point
+ *   gC[2^E E], gS[2^E E], gsum[2]   the forward-KL statistics per (state,
+ *                 cut edge) (ftr_abi.hpp LearnLaunch), summed in fixed point
  *   dY[max(32, 2^E) x 32]         dL/dY of the network outputs
  * No reference counterpart (learned proposal). */
 int ftr_learner_statistics(ftr_ctx* ctx, ftr_learner* l, uint64_t samples, uint64_t seed, double* cw, double* ly,
@@ -258,7 +258,8 @@ int ftr_learner_statistics(ftr_ctx* ctx, ftr_learner* l, uint64_t samples, uint6
  * statistics weight each path by c = |f| p_trop/q), 1 the estimator's
  * variance E_q[(f p_trop/q)^2] (c = (|f| p_trop/q)^2). */
 int ftr_learner_set_objective(ftr_learner* l, int objective);
-/* Current tables, host arrays: lw, lratio (2^E x E); lZ, lia, lrad, ldiff (2^E). */
+/* Current tables, host arrays: lw, lratio, lia, lrad, ldiff (2^E x E, per
+ * state and cut edge); lZ (2^E). */
 int ftr_learner_tables(ftr_ctx* ctx, ftr_learner* l, double* lw, double* lZ, double* lratio, double* lia,
                        double* lrad, double* ldiff);
 /* Integrate with the learned proposal: moments of R * p_trop/q. */

@@ -124,9 +124,18 @@ def tables(Y, E, w, Jr, r):
     for e in range(E):
         has = ((g >> e) & 1) == 1
         lratio[has, e] = (lZ[has] * w[g[has] ^ (1 << e)]) / (Jr[has] * lw[has, e])
-    sg = SMAX * np.tanh(Y[:n, E].astype(np.float64) / SMAX)
-    a = r * np.exp(sg)
-    return dict(lw=lw, lZ=lZ, lratio=lratio, lia=1.0 / a, lrad=-sg, ldiff=r - a)
+    # radial exponent after cutting e from g: a = r(g\e) e^sigma(g,e), sigma
+    # from column E+e; zero where no radius follows (e not in g, or g\e empty)
+    lia, lrad, ldiff = np.zeros((n, E)), np.zeros((n, E)), np.zeros((n, E))
+    for e in range(E):
+        has = (((g >> e) & 1) == 1) & (g != (1 << e))
+        c = g[has] ^ (1 << e)
+        sg = SMAX * np.tanh(Y[:n][has, E + e].astype(np.float64) / SMAX)
+        a = r[c] * np.exp(sg)
+        lia[has, e] = 1.0 / a
+        lrad[has, e] = -sg
+        ldiff[has, e] = r[c] - a
+    return dict(lw=lw, lZ=lZ, lratio=lratio, lia=lia, lrad=lrad, ldiff=ldiff)
 
 
 def replay(t, E, U):
@@ -159,23 +168,25 @@ def replay(t, E, U):
         c = g ^ (1 << sel)
         X[rows, sel] = XX
         if j < E - 1:
-            lY = t["lia"][c] * np.log(1.0 - U[:, 2 * j + 1])
+            lY = t["lia"][g, sel] * np.log(1.0 - U[:, 2 * j + 1])
             lX = lX + lY
             XX = np.exp(lX)
-            lrad = lrad + t["lrad"][c] + t["ldiff"][c] * lY
+            lrad = lrad + t["lrad"][g, sel] + t["ldiff"][g, sel] * lY
         g = c
     return X, ratio * np.exp(lrad), order
 
 
 def weight_from_path(X, order, t, w, Jr, r):
     """p_trop(path)/q(path) from first principles: cut probabilities
-    w(g\\e)/J(g) against lw/lZ, radial densities r y^(r-1) against
-    a y^(a-1) with a = 1/lia; y from consecutive X along the path."""
+    w(g\\e)/J(g) against lw/lZ, radial densities r y^(r-1) (r of the state
+    the previous cut reached) against a y^(a-1) with a = 1/lia of that
+    previous cut; y from consecutive X along the path."""
     n, E = X.shape
     logw = np.zeros(n)
     g = np.full(n, (1 << E) - 1, dtype=np.int64)
     rows = np.arange(n)
     prevX = np.ones(n)
+    pg, pe = g, order[:, 0]
     for j in range(E):
         e = order[:, j]
         c = g ^ (1 << e)
@@ -183,10 +194,11 @@ def weight_from_path(X, order, t, w, Jr, r):
         if j > 0:
             y = X[rows, e] / prevX
             ly = np.log(y)
-            a = 1.0 / t["lia"][g]
+            a = 1.0 / t["lia"][pg, pe]
             rr = r[g]
             logw += (np.log(rr) + (rr - 1) * ly) - (np.log(a) + (a - 1) * ly)
         prevX = X[rows, e]
+        pg, pe = g, e
         g = c
     return np.exp(logw)
 

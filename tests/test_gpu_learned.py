@@ -91,12 +91,11 @@ def test_tables_and_sampler_replay(engine, oracle, name):
         w = Jr / r
     t = lr.tables()
     tr = LR.tables(Y, p.E, w, Jr, r)
-    n_states = 1 << p.E
-    used = np.arange(n_states) != n_states - 1    # a/lia of the full graph are never used
     for k in ("lw", "lZ", "lratio"):
         np.testing.assert_allclose(t[k], tr[k], rtol=1e-13, atol=0, err_msg=k)
-    for k in ("lia", "lrad", "ldiff"):
-        np.testing.assert_allclose(t[k][used], tr[k][used], rtol=1e-13, atol=1e-15, err_msg=k)
+    for k in ("lia", "lrad", "ldiff"):   # per (state, cut edge); zero where no radius follows
+        assert t[k].shape == (1 << p.E, p.E)
+        np.testing.assert_allclose(t[k], tr[k], rtol=1e-13, atol=1e-15, err_msg=k)
     n, first, seed = 20000, 12345, 9
     E = p.E
     Xd = torch.zeros(n * E, dtype=torch.float64, device="cuda")
@@ -214,7 +213,7 @@ def test_training_statistics_and_gradient(engine, name, objective):
     c, ly, path = st["c"], st["ly"], st["path"].astype(np.uint64)
     assert st["gsum"][1] == N
     ok = (c != 0) & np.isfinite(c)
-    C = np.zeros((n, E)); S = np.zeros(n); nC = np.zeros((n, E)); nS = np.zeros(n)
+    C = np.zeros((n, E)); S = np.zeros((n, E)); nC = np.zeros((n, E)); nS = np.zeros((n, E))
     g = np.full(N, n - 1, np.int64)
     M = np.abs(c[ok]).max()
     for j in range(E):
@@ -223,10 +222,10 @@ def test_training_statistics_and_gradient(engine, name, objective):
         np.add.at(nC, (g[ok], e[ok]), 1.0)
         cg = g ^ (1 << e)
         if j < E - 1:
-            sv = c * (1.0 + ly[j] / t["lia"][cg])
+            sv = c * (1.0 + ly[j] / t["lia"][g, e])
             good = ok & np.isfinite(sv)
-            np.add.at(S, cg[good], sv[good])
-            np.add.at(nS, cg[good], 1.0)
+            np.add.at(S, (g[good], e[good]), sv[good])
+            np.add.at(nS, (g[good], e[good]), 1.0)
             M = max(M, np.abs(sv[good]).max())
         g = cg
     k = min(max(62 - np.frexp(M)[1] - np.frexp(float(N))[1], -1000), 1000)
@@ -244,8 +243,9 @@ def test_training_statistics_and_gradient(engine, name, objective):
     th = np.tanh(Y[:, :E] / 8.0)
     ref = np.zeros((n, 32))
     ref[:, :E] = np.where(present, -(gC - q * visits[:, None]) / Ssum * (1.0 - th * th), 0.0)
-    ts = np.tanh(Y[:, E] / 0.6)
-    ref[:, E] = -gS / Ssum * (1.0 - ts * ts)
+    ts = np.tanh(Y[:, E:2 * E] / 0.6)
+    radial = present & (np.arange(n)[:, None] != (1 << np.arange(E))[None, :])
+    ref[:, E:2 * E] = np.where(radial, -gS / Ssum * (1.0 - ts * ts), 0.0)
     dY = st["dY"][:n].astype(np.float64)
     scale = np.abs(ref).max()
     assert scale > 0
