@@ -43,7 +43,7 @@ constexpr size_t WS_FLOATS = (size_t)MAX_SPLITS * H * H; /* their workspace */
 
 struct MlpModule {
   hipModule_t mod = nullptr;
-  hipFunction_t gbb, gfb, silu_bwd, colsum, split_sum, adam, fill, fill_f, to_bf16, tables, tgrad;
+  hipFunction_t gbb, gfb, colsum, split_sum, adam, fill, fill_f, to_bf16, tables, tgrad;
 };
 
 MlpModule load_mlp(ftr_ctx* c) {
@@ -51,7 +51,6 @@ MlpModule load_mlp(ftr_ctx* c) {
   m.mod = load_module_raw(c, ftr_embedded_mlp_source);
   HIPCHK(hipModuleGetFunction(&m.gbb, m.mod, "mlp_gemm_bb"));
   HIPCHK(hipModuleGetFunction(&m.gfb, m.mod, "mlp_gemm_fb"));
-  HIPCHK(hipModuleGetFunction(&m.silu_bwd, m.mod, "mlp_silu_bwd"));
   HIPCHK(hipModuleGetFunction(&m.colsum, m.mod, "mlp_colsum"));
   HIPCHK(hipModuleGetFunction(&m.split_sum, m.mod, "mlp_split_sum"));
   HIPCHK(hipModuleGetFunction(&m.adam, m.mod, "mlp_adam"));
@@ -125,11 +124,18 @@ void split_sum(ftr_ctx* c, ftr_learner* l, float* out, uint64_t n, int splits) {
   launch(l->mlp.split_sum, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
 }
 
-/* C = act(A B + bias), operands addressed by strides (see ftr_mlp.hpp) */
+/* C = act(A B + bias), operands addressed by strides (see ftr_mlp.hpp);
+ * act 2: C = (A B) * silu'(zs) */
 void gemm(ftr_ctx* c, ftr_learner* l, bool a_float, const void* A, uint64_t sai, uint64_t sak, const uint16_t* B,
           uint64_t sbk, uint64_t sbj, const float* bias, float* C, uint64_t ldc, uint16_t* Cb, uint64_t ldcb, int M,
-          int N, int K, int act, bool split_k = false) {
-  if (M % 32 || N % 32 || K % 8) throw Error(FTR_ERR_INVALID, "network GEMM shape not tiled");
+          int N, int K, int act, bool split_k = false, const float* zs = nullptr) {
+  if (M % 32 || N % 32 || K % 32) throw Error(FTR_ERR_INVALID, "network GEMM shape not tiled");
+  /* the kernel stages operands in 8-element vectors along their unit stride */
+  auto vec_ok = [](const void* p, uint64_t s0, uint64_t s1) {
+    return ((uintptr_t)p % 16 == 0) && ((s0 == 1 && s1 % 8 == 0) || (s1 == 1 && s0 % 8 == 0));
+  };
+  if (!vec_ok(A, sai, sak) || !vec_ok(B, sbk, sbj)) throw Error(FTR_ERR_INVALID, "network GEMM operand layout");
+  if ((act == 2) != (zs != nullptr)) throw Error(FTR_ERR_INVALID, "network GEMM: SiLU backward needs Z");
   /* long reductions (weight gradients over all states) split K over at
    * most MAX_SPLITS blocks; their partial products go to the workspace and
    * are added in a fixed order (reproducible, no atomics) */
@@ -137,11 +143,12 @@ void gemm(ftr_ctx* c, ftr_learner* l, bool a_float, const void* A, uint64_t sai,
   if (split_k && K >= 256 && !Cb && !bias && !act) {
     if (ldc != (uint64_t)N || (size_t)MAX_SPLITS * M * N > WS_FLOATS)
       throw Error(FTR_ERR_INVALID, "network GEMM split does not fit the workspace");
-    kc = std::max(64, (((K + MAX_SPLITS - 1) / MAX_SPLITS) + 7) / 8 * 8);
+    kc = std::max(64, (((K + MAX_SPLITS - 1) / MAX_SPLITS) + 63) / 64 * 64);
     splits = (K + kc - 1) / kc;
   }
   float* dst = splits > 1 ? l->d_ws : C;
-  void* args[] = {(void*)&A, &sai, &sak, (void*)&B, &sbk, &sbj, (void*)&bias, &dst, &ldc, &Cb, &ldcb, &K, &act, &kc};
+  void* args[] = {(void*)&A, &sai, &sak, (void*)&B, &sbk, &sbj, (void*)&bias, &dst, &ldc, &Cb, &ldcb, &K, &act, &kc,
+                  (void*)&zs};
   HIPCHK(hipModuleLaunchKernel(a_float ? l->mlp.gfb : l->mlp.gbb, (unsigned)(N / 32), (unsigned)(M / 32),
                                (unsigned)splits, 64, 1, 1, 0, c->stream, args, nullptr));
   if (splits > 1) split_sum(c, l, C, (uint64_t)M * N, splits);
@@ -185,33 +192,25 @@ void colsum(ftr_ctx* c, ftr_learner* l, const float* dZ, float* db, int rows, in
   split_sum(c, l, db, (uint64_t)cols, nchunks);
 }
 
-void silu_bwd(ftr_ctx* c, ftr_learner* l, const float* dH, const float* Z, float* dZ, uint64_t n) {
-  void* args[] = {(void*)&dH, (void*)&Z, &dZ, &n};
-  launch(l->mlp.silu_bwd, (unsigned)((n + 255) / 256), 1, 256, c->stream, args);
-}
-
 /* parameter gradient of sum(dY * Y) into d_grad (forward must have run) */
 void backward(ftr_ctx* c, ftr_learner* l) {
   const uint16_t* W = l->d_parb;
   float* G = l->d_grad;
   const int M = l->M;
-  /* layer 4: dW4 = dY^T H3, db4 = colsum dY, dH3 = dY W4 */
+  /* layer 4: dW4 = dY^T H3, db4 = colsum dY, dZ3 = (dY W4) silu'(Z3) */
   gemm(c, l, true, l->d_dY, 1, NOUT, l->d_H3, H, 1, nullptr, G + OFF_W4, H, nullptr, 0, NOUT, H, M, 0, true);
   colsum(c, l, l->d_dY, G + OFF_B4, M, NOUT);
-  gemm(c, l, true, l->d_dY, NOUT, 1, W + OFF_W4, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, NOUT, 0);
-  const uint64_t n = (uint64_t)M * H;
-  /* layer 3 */
-  silu_bwd(c, l, l->d_dH, l->d_Z3, l->d_dZ, n);
+  gemm(c, l, true, l->d_dY, NOUT, 1, W + OFF_W4, H, 1, nullptr, l->d_dZ, H, nullptr, 0, M, H, NOUT, 2, false,
+       l->d_Z3);
+  /* layer 3 (dZ3 in d_dZ; dZ2 to d_dH) */
   gemm(c, l, true, l->d_dZ, 1, H, l->d_H2, H, 1, nullptr, G + OFF_W3, H, nullptr, 0, H, H, M, 0, true);
   colsum(c, l, l->d_dZ, G + OFF_B3, M, H);
-  gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W3, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 0);
-  /* layer 2 */
-  silu_bwd(c, l, l->d_dH, l->d_Z2, l->d_dZ, n);
-  gemm(c, l, true, l->d_dZ, 1, H, l->d_H1, H, 1, nullptr, G + OFF_W2, H, nullptr, 0, H, H, M, 0, true);
-  colsum(c, l, l->d_dZ, G + OFF_B2, M, H);
-  gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W2, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 0);
-  /* layer 1 */
-  silu_bwd(c, l, l->d_dH, l->d_Z1, l->d_dZ, n);
+  gemm(c, l, true, l->d_dZ, H, 1, W + OFF_W3, H, 1, nullptr, l->d_dH, H, nullptr, 0, M, H, H, 2, false, l->d_Z2);
+  /* layer 2 (dZ2 in d_dH; dZ1 to d_dZ) */
+  gemm(c, l, true, l->d_dH, 1, H, l->d_H1, H, 1, nullptr, G + OFF_W2, H, nullptr, 0, H, H, M, 0, true);
+  colsum(c, l, l->d_dH, G + OFF_B2, M, H);
+  gemm(c, l, true, l->d_dH, H, 1, W + OFF_W2, H, 1, nullptr, l->d_dZ, H, nullptr, 0, M, H, H, 2, false, l->d_Z1);
+  /* layer 1 (dZ1 in d_dZ) */
   gemm(c, l, true, l->d_dZ, 1, H, l->d_X0b, NIN, 1, nullptr, G + OFF_W1, NIN, nullptr, 0, H, NIN, M, 0, true);
   colsum(c, l, l->d_dZ, G + OFF_B1, M, H);
 }

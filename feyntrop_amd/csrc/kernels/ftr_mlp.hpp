@@ -3,12 +3,14 @@
  * counterpart).  A 4-layer MLP (input 32, three hidden layers of 128 SiLU
  * units, output 32) in BF16 with FP32 accumulation:
  *   Z1 = X W1^T + b1, H1 = silu(Z1); Z2, H2; Z3, H3; Y = H3 W4^T + b4.
- * Every dense product is v_mfma_f32_32x32x8_bf16 (one wave per 32x32 output
- * tile); forward and backward use the same kernel with strided operands.
+ * Every dense product is gfx950's v_mfma_f32_32x32x16_bf16 (one wave per
+ * 32x32 output tile, operands staged through LDS); forward and backward use
+ * the same kernel with strided operands.
  *
- * MFMA lane layout used (checked on the device): for lane l,
- *   A (32x8):  A[l%32][4(l/32)+t], t = 0..3
- *   B (8x32):  B[4(l/32)+t][l%32]
+ * MFMA lane layout used (checked on the device against a numpy product):
+ * for lane l,
+ *   A (32x16): A[l%32][8(l/32)+t], t = 0..7
+ *   B (16x32): B[8(l/32)+t][l%32]
  *   C (32x32): acc v -> C[8(v/4) + 4(l/32) + v%4][l%32]
  */
 #pragma once
@@ -24,7 +26,7 @@ typedef __hip_internal::uint64_t uint64_t;
 
 namespace ftrm {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 /* round-to-nearest-even float -> bf16 bits */
@@ -42,55 +44,155 @@ __device__ inline float dsilu(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-/* Operand access: element (i, k) of A lives at A[i*si + k*sk]; AT is the
- * storage type (unsigned short = bf16 bits, or float converted on load). */
-template <class AT>
-__device__ inline short ld_bf(const AT* p, size_t off);
+/* Operands: element (i, k) of A lives at A[i*si + k*sk], with one of the
+ * two strides 1 and the other a multiple of 8; T is the storage type
+ * (unsigned short = bf16 bits, or float rounded to bf16).  Raw<T> holds 8
+ * consecutive elements as loaded (16-byte aligned); pack8 turns them into 8
+ * bf16 in a uint4.  Loading and packing are separate so that a lane's loads
+ * can all be in flight before the first conversion. */
+template <class T>
+struct Raw;
 template <>
-__device__ inline short ld_bf<unsigned short>(const unsigned short* p, size_t off) {
-  return (short)p[off];
-}
+struct Raw<unsigned short> {
+  uint4 v;
+  __device__ inline void load(const unsigned short* p) { v = *reinterpret_cast<const uint4*>(p); }
+  __device__ inline uint4 pack8() const { return v; }
+};
 template <>
-__device__ inline short ld_bf<float>(const float* p, size_t off) {
-  return (short)f2bf(p[off]);
+struct Raw<float> {
+  float4 a, b;
+  __device__ inline void load(const float* p) {
+    a = reinterpret_cast<const float4*>(p)[0];
+    b = reinterpret_cast<const float4*>(p)[1];
+  }
+  __device__ inline uint4 pack8() const {
+    uint4 r;
+    r.x = (uint32_t)f2bf(a.x) | ((uint32_t)f2bf(a.y) << 16);
+    r.y = (uint32_t)f2bf(a.z) | ((uint32_t)f2bf(a.w) << 16);
+    r.z = (uint32_t)f2bf(b.x) | ((uint32_t)f2bf(b.y) << 16);
+    r.w = (uint32_t)f2bf(b.z) | ((uint32_t)f2bf(b.w) << 16);
+    return r;
+  }
+};
+
+constexpr int KC = 64;       /* K chunk staged through LDS */
+constexpr int KP = KC + 8;   /* padded LDS row (144 B: 16-byte reads spread over the banks) */
+
+/* Staging of a K chunk: dst[r][k] = src(r, k0 + k) for r < 32,
+ * k < 16 NV (NV = 1, 2 or 4 vectors per lane), element (r, k) at
+ * src[r*sr + k*sk] with sk == 1 or sr == 1.  load_n reads, store_n writes
+ * LDS: a chunk issues the loads of both operands before its first LDS
+ * write, so it costs one memory round trip. */
+template <int NV, class T>
+__device__ inline void load_n(Raw<T> (&v)[NV], const T* src, size_t sr, size_t sk, int k0) {
+  const int l = threadIdx.x;
+  if (sk == 1) {
+    /* vectors along k: vector q covers row q / (2 NV), k = 8 (q % (2 NV)) */
+    constexpr int SH = NV == 4 ? 3 : (NV == 2 ? 2 : 1);
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+      const int q = l + 64 * t;
+      v[t].load(src + (size_t)(q >> SH) * sr + (size_t)(k0 + 8 * (q & ((1 << SH) - 1))));
+    }
+  } else {
+    /* vectors along the rows: vector q covers rows 8 (q % 4) .. +8 at k = q / 4 */
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+      const int q = l + 64 * t;
+      v[t].load(src + (size_t)(8 * (q & 3)) + (size_t)(k0 + (q >> 2)) * sk);
+    }
+  }
 }
 
-/* C[M x N] = act(A[M x K] B[K x N] + bias)  (M, N multiples of 32, K of 8).
+template <int NV, class T>
+__device__ inline void store_n(unsigned short (*dst)[KP], const Raw<T> (&v)[NV], size_t sk) {
+  const int l = threadIdx.x;
+  if (sk == 1) {
+    constexpr int SH = NV == 4 ? 3 : (NV == 2 ? 2 : 1);
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+      const int q = l + 64 * t;
+      *reinterpret_cast<uint4*>(&dst[q >> SH][8 * (q & ((1 << SH) - 1))]) = v[t].pack8();
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+      const int q = l + 64 * t;
+      const int r0 = 8 * (q & 3), k = q >> 2;
+      const uint4 pv = v[t].pack8();
+      const uint32_t w[4] = {pv.x, pv.y, pv.z, pv.w};
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        dst[r0 + 2 * u][k] = (unsigned short)(w[u] & 0xffffu);
+        dst[r0 + 2 * u + 1][k] = (unsigned short)(w[u] >> 16);
+      }
+    }
+  }
+}
+
+/* one K chunk of 16 NV: stage both operands, then NV MFMA steps */
+template <int NV, class AT, class BT>
+__device__ inline void gemm_chunk(f32x16& acc, unsigned short (*sA)[KP], unsigned short (*sB)[KP], const AT* Ab,
+                                  size_t sai, size_t sak, const BT* Bb, size_t sbj, size_t sbk, int k0) {
+  Raw<AT> ra[NV];
+  Raw<BT> rb[NV];
+  load_n<NV>(ra, Ab, sai, sak, k0);
+  load_n<NV>(rb, Bb, sbj, sbk, k0);
+  store_n<NV>(sA, ra, sak);
+  store_n<NV>(sB, rb, sbk);
+  __syncthreads();
+  const int l = threadIdx.x;
+  const int r = l & 31, kq = 8 * (l >> 5);
+#pragma unroll
+  for (int kk = 0; kk < 16 * NV; kk += 16) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[r][kk + kq]);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[r][kk + kq]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+  __syncthreads();
+}
+
+/* C[M x N] = act(A[M x K] B[K x N] + bias)  (M, N, K multiples of 32).
  * Writes the FP32 result to C (if non-null) and its bf16 rounding after the
- * activation to Cb (if non-null).  act: 0 identity, 1 SiLU.  One wave per
- * output tile; grid = (N/32, M/32, splits) blocks of 64 threads.  With
+ * activation to Cb (if non-null).  act: 0 identity, 1 SiLU, 2 the SiLU
+ * backward: C = (A B) * silu'(Z) with Z = zs (same layout as C).  One wave
+ * per output tile; grid = (N/32, M/32, splits) blocks of 64 threads.  With
  * splits > 1 block z covers K rows [z kc, (z+1) kc) and stores its partial
  * product to slice z of C (slices of M x ldc floats; bias, act and Cb
  * unused); mlp_split_sum then adds the slices in a fixed order. */
 template <class AT, class BT>
 __device__ inline void gemm_tile(const AT* A, size_t sai, size_t sak, const BT* B, size_t sbk, size_t sbj,
                                  const float* bias, float* C, size_t ldc, unsigned short* Cb, size_t ldcb,
-                                 int K, int act, int kc) {
+                                 int K, int act, int kc, const float* zs) {
+  __shared__ __align__(16) unsigned short sA[32][KP]; /* [row i][k] */
+  __shared__ __align__(16) unsigned short sB[32][KP]; /* [column j][k] */
   const int l = threadIdx.x;
   const int ti = blockIdx.y, tj = blockIdx.x;
-  const int ar = ti * 32 + (l & 31), bc = tj * 32 + (l & 31), kq = 4 * (l >> 5);
   const bool split = gridDim.z > 1;
   const int kbeg = split ? (int)blockIdx.z * kc : 0;
   const int kend = split ? (kbeg + kc < K ? kbeg + kc : K) : K;
+  const AT* Ab = A + (size_t)ti * 32 * sai;
+  const BT* Bb = B + (size_t)tj * 32 * sbj;
+  const int r = l & 31;
   f32x16 acc;
 #pragma unroll
   for (int v = 0; v < 16; v++) acc[v] = 0.0f;
-#pragma unroll 4
-  for (int k0 = kbeg; k0 < kend; k0 += 8) {
-    bf16x4 a, b;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      a[t] = ld_bf<AT>(A, (size_t)ar * sai + (size_t)(k0 + kq + t) * sak);
-      b[t] = ld_bf<BT>(B, (size_t)(k0 + kq + t) * sbk + (size_t)bc * sbj);
-    }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  for (int k0 = kbeg; k0 < kend; k0 += KC) {
+    const int kn = (kend - k0 < KC) ? kend - k0 : KC;   /* 16, 32 or 64 */
+    if (kn == 64) gemm_chunk<4>(acc, sA, sB, Ab, sai, sak, Bb, sbj, sbk, k0);
+    else if (kn == 32) gemm_chunk<2>(acc, sA, sB, Ab, sai, sak, Bb, sbj, sbk, k0);
+    else gemm_chunk<1>(acc, sA, sB, Ab, sai, sak, Bb, sbj, sbk, k0);
   }
 #pragma unroll
   for (int v = 0; v < 16; v++) {
     const int i = ti * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
-    const int j = bc;
+    const int j = tj * 32 + r;
     if (split) {
       C[(size_t)blockIdx.z * gridDim.y * 32 * ldc + (size_t)i * ldc + j] = acc[v];
+      continue;
+    }
+    if (act == 2) {
+      C[(size_t)i * ldc + j] = acc[v] * dsilu(zs[(size_t)i * ldc + j]);
       continue;
     }
     const float z = acc[v] + (bias ? bias[j] : 0.0f);
@@ -104,21 +206,15 @@ __device__ inline void gemm_tile(const AT* A, size_t sai, size_t sak, const BT* 
 extern "C" __global__ void __launch_bounds__(64)
     mlp_gemm_bb(const unsigned short* A, uint64_t sai, uint64_t sak, const unsigned short* B, uint64_t sbk,
                 uint64_t sbj, const float* bias, float* C, uint64_t ldc, unsigned short* Cb, uint64_t ldcb, int K,
-                int act, int kc) {
-  ftrm::gemm_tile<unsigned short, unsigned short>(A, sai, sak, B, sbk, sbj, bias, C, ldc, Cb, ldcb, K, act, kc);
+                int act, int kc, const float* zs) {
+  ftrm::gemm_tile<unsigned short, unsigned short>(A, sai, sak, B, sbk, sbj, bias, C, ldc, Cb, ldcb, K, act, kc, zs);
 }
 
 extern "C" __global__ void __launch_bounds__(64)
     mlp_gemm_fb(const float* A, uint64_t sai, uint64_t sak, const unsigned short* B, uint64_t sbk, uint64_t sbj,
                 const float* bias, float* C, uint64_t ldc, unsigned short* Cb, uint64_t ldcb, int K, int act,
-                int kc) {
-  ftrm::gemm_tile<float, unsigned short>(A, sai, sak, B, sbk, sbj, bias, C, ldc, Cb, ldcb, K, act, kc);
-}
-
-/* dZ = dH * silu'(Z) elementwise (n values). */
-extern "C" __global__ void mlp_silu_bwd(const float* dH, const float* Z, float* dZ, uint64_t n) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dZ[i] = dH[i] * ftrm::dsilu(Z[i]);
+                int kc, const float* zs) {
+  ftrm::gemm_tile<float, unsigned short>(A, sai, sak, B, sbk, sbj, bias, C, ldc, Cb, ldcb, K, act, kc, zs);
 }
 
 /* part[y][j] = sum of dZ[i][j] over the rows i of chunk y (rows x cols,
