@@ -200,6 +200,21 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
   /* in-step row prefetch of the sampler (ftr_device.hpp sample_point):
    * generic deformed-path kernels */
   s << "  static constexpr bool PF = " << ((v.mode == 2 && !v.cparams) ? "true" : "false") << ";\n";
+  {
+    /* software-pipelined sample loop (ftr_device.hpp integrate_body) */
+    static const int pipe = [] {
+      const char* e = getenv("FEYNTROP_AMD_PIPE");  /* =0 / =1 override */
+      return e ? atoi(e) : -1;
+    }();
+    /* measured per 1e8 samples: real path cubic5 -2.2 % (specialised) and
+     * -2.6 % (generic), K4 wheel -3.4 % (specialised); generic deformed
+     * kernels 0 to -2.7 % (triple ladder); deformed kernels specialised to
+     * a kinematic point +0 to +5 % (2-loop tutorial) -- off for those;
+     * the batched bucket kernels (many graphs in one kernel, not
+     * instantiated here) +3 % on the 4096-point scan -- off */
+    const bool on = pipe >= 0 ? pipe == 1 : (instantiate && (v.mode != 2 || !v.cparams));
+    s << "  static constexpr bool PIPE = " << (on ? "true" : "false") << ";\n";
+  }
   s << (v.cparams ? "  using PRM = " + gname + "P;\n" : std::string("  using PRM = ftr::DevParams;\n"));
   arr(s, "hi", g.hi);
   arr(s, "lo", g.lo);

@@ -188,7 +188,7 @@ __host__ __device__ inline cplx cpow_mW(const PT& p, cplx z) {
  * in the same order, so the cut is the same; measured faster on the real
  * path (cubic5 -7 % generic, -12 % specialised; K4 wheel -1 %) and slower
  * on the deformed one (double box generic +2 %). */
-template <class G, bool JSUM = false, bool PF = false, class TV>
+template <class G, bool JSUM = false, bool PF = false, bool UNR = false, class TV>
 __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& lpsi_tr,
                                     double& lG) {
   /* The radius product x = prod (1-U)^(1/r) is carried as its logarithm, so
@@ -216,7 +216,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
 #pragma unroll
       for (int q = 0; q < E; q++) rv[q] = row[q];
     }
-    for (int j = 0; j < E; j++) {
+    auto step = [&](int j) {
       const double R = rng.uniform() * JJr;
       double T = 0.0;
       int cnt = 0;
@@ -250,6 +250,15 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
         lX += ir * log_pos(1. - U);
         XX = exp(lX);
       }
+    };
+    /* UNR: the steps unrolled, so that the sampler is straight-line code
+     * that the scheduler can interleave with other work (integrate_body's
+     * pipelined loop) */
+    if constexpr (UNR) {
+#pragma unroll
+      for (int j = 0; j < E; j++) step(j);
+    } else {
+      for (int j = 0; j < E; j++) step(j);
     }
 #pragma unroll
     for (int k = 0; k < E; k++) X[k] = tv.xs[k * G::BLK];
@@ -265,7 +274,7 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
   double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0;
 #pragma unroll
   for (int e = 0; e < E; e++) X[e] = 0.0;
-  for (int j = 0; j < E; j++) {
+  auto step = [&](int j) {
     int e = 0;
     if constexpr (TV::TM == 3 && JSUM) {
       const double* row = tv.row(g);
@@ -339,6 +348,12 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
       lX += ir * log_pos(1. - U);
       XX = exp(lX);
     }
+  };
+  if constexpr (UNR) {
+#pragma unroll
+    for (int j = 0; j < E; j++) step(j);
+  } else {
+    for (int j = 0; j < E; j++) step(j);
   }
   if constexpr (TV::TM == 3) {
 #pragma unroll
@@ -911,9 +926,25 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   double acc[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+  /* G::PIPE: software-pipelined -- sample i+1 is drawn (sampler unrolled)
+   * right after sample i's point is taken, so that its dependent table
+   * loads overlap sample i's integrand; the lane's last draw is discarded.
+   * The samples and their order are those of the plain loop. */
+  double Xn[E], lpn = 0.0, lGn = 0.0;
+  if constexpr (G::PIPE) {
+    if (n > 0) sample_point<G, MODE != 2, G::PF, true>(rng, tv, Xn, lpn, lGn);
+  }
   for (uint64_t j = n; j > 0; j--) {
     double X[E], lpsi, lG;
-    sample_point<G, MODE != 2, G::PF>(rng, tv, X, lpsi, lG);
+    if constexpr (G::PIPE) {
+#pragma unroll
+      for (int e = 0; e < E; e++) X[e] = Xn[e];
+      lpsi = lpn;
+      lG = lGn;
+      sample_point<G, MODE != 2, G::PF, true>(rng, tv, Xn, lpn, lGn);
+    } else {
+      sample_point<G, MODE != 2, G::PF>(rng, tv, X, lpsi, lG);
+    }
     cplx out[K];
     if (!integrand<G, K, MODE, PT, G::DDL>(p, X, log_scale(p, lpsi, lG), out, dds)) {
       acc[1] += 1.0;
