@@ -105,6 +105,7 @@ def lib():
     L.ftr_batch_error.restype = C.c_char_p
     L.ftr_batch_error.argtypes = [vp, C.c_int]
     L.ftr_set_specialized.argtypes = [vp, vp, C.c_int]
+    L.ftr_use_cached_specialization.argtypes = [vp, vp]
     L.ftr_get_specialized.argtypes = [vp, C.POINTER(C.c_int)]
     L.ftr_precompile_specialized.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_precompile_learned.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
@@ -351,6 +352,15 @@ class Engine:
         rc = lib().ftr_set_specialized(self.h, p.h, 1 if on else 0)
         if rc:
             raise self._err(rc)
+
+    def use_cached_specialization(self, p):
+        """The feyntrop binary's kernel choice: p's specialised kernel when it
+        is already cached (FEYNTROP_AMD_SPECIALIZE=1 always, =0 never);
+        returns whether p now runs it."""
+        rc = lib().ftr_use_cached_specialization(self.h, p.h)
+        if rc:
+            raise self._err(rc)
+        return self.specialized(p)
 
     def specialized(self, p):
         on = C.c_int()

@@ -199,6 +199,9 @@ extern "C" int ftr_run_json(ftr_ctx* ctx, const char* json_in, char** json_out, 
       const ftr_input in = c.in();
       rc = ftr_prepare(ctx, &in, &prob);
       if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
+      /* the kernel specialised to this kinematic point when it is cached */
+      rc = ftr_use_cached_specialization(ctx, prob);
+      if (rc != FTR_OK) throw Error(rc, ftr_last_error(ctx));
       /* regime / GP log lines were collected during preprocessing */
       for (const std::string& l : ftr_problem_log(prob)) log += l + "\n";
       /* optional addition: "proposal": "learned" trains the learned proposal

@@ -142,6 +142,7 @@ inline bool dd_in_lds(const Variant& v, int E) {
   return v.mode == 2 && v.tmode == 3 && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
 }
 int fail(ftr_ctx* c, int code, const std::string& msg);
+
 /* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */
 void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& res);
 /* build the row tables of table mode 3 on the device (jtable_gpu.cpp):

@@ -243,8 +243,9 @@ static const std::vector<std::string>& rtc_options() {
   return o;
 }
 
-/* Compile (or fetch from the cache) the code object for a source. */
-std::string code_object(const std::string& src, std::string* path_out) {
+/* The cache path of a source's code object: the source, the compile
+ * options and the hipRTC version hashed. */
+static std::string code_object_path(const std::string& src) {
   std::string key = src;
   for (auto& o : rtc_options()) key += "\n" + o;
   int rtcver = 0, rtcmin = 0;
@@ -252,8 +253,13 @@ std::string code_object(const std::string& src, std::string* path_out) {
   key += "\nhiprtc " + std::to_string(rtcver) + "." + std::to_string(rtcmin);
   char name[64];
   snprintf(name, sizeof name, "ftr_%016llx.co", (unsigned long long)fnv1a(key));
+  return cache_dir() + "/" + name;
+}
+
+/* Compile (or fetch from the cache) the code object for a source. */
+std::string code_object(const std::string& src, std::string* path_out) {
   const std::string dir = cache_dir();
-  const std::string path = dir + "/" + name;
+  const std::string path = code_object_path(src);
   if (path_out) *path_out = path;
   {
     std::ifstream f(path, std::ios::binary);
@@ -1686,6 +1692,20 @@ int ftr_set_specialized(ftr_ctx* c, ftr_problem* p, int on) {
     return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
   }
   return FTR_OK;
+}
+
+int ftr_use_cached_specialization(ftr_ctx* c, ftr_problem* p) {
+  if (!c || !p || !p->m.f_int || p->var.cparams) return FTR_OK;
+  const char* e = getenv("FEYNTROP_AMD_SPECIALIZE");
+  if (e && !strcmp(e, "0")) return FTR_OK;
+  const bool always = e && !strcmp(e, "1");
+  if (!always) {
+    Variant v = p->var;
+    v.cparams = 1;
+    std::ifstream f(code_object_path(full_source(p->prep, v)), std::ios::binary);
+    if (!f) return FTR_OK;
+  }
+  return ftr_set_specialized(c, p, 1);
 }
 
 int ftr_set_jtable_mode(ftr_ctx* c, int mode) {

@@ -191,6 +191,12 @@ int ftr_set_variant(ftr_ctx* ctx, ftr_problem* p, int table_mode, int block);
  * samples, same estimator; for long runs of one point. */
 int ftr_set_specialized(ftr_ctx* ctx, ftr_problem* p, int on);
 int ftr_get_specialized(const ftr_problem* p, int* on);
+/* The drop-in binary's kernel choice (no reference counterpart; the feyntrop
+ * CLI calls it after ftr_prepare): switches p to its specialised kernel
+ * when that kernel's code object is already in the cache, so no compile is
+ * paid; environment FEYNTROP_AMD_SPECIALIZE=1 specialises always (compiling
+ * if needed), =0 never.  FTR_OK, or the error of ftr_set_specialized. */
+int ftr_use_cached_specialization(ftr_ctx* ctx, ftr_problem* p);
 /* Compile the specialised kernel of a problem into the cache without a GPU. */
 int ftr_precompile_specialized(const ftr_input* in, char* err, int errlen);
 int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block);

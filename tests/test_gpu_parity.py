@@ -387,6 +387,32 @@ def test_cli_learned_proposal(golden_fixtures):
             assert abs(a - b) <= 3 * np.hypot(da, db), (k, q, a, da, b, db)
 
 
+def test_cli_uses_cached_specialized_kernel(engine):
+    """The binary runs the kernel specialised to the kinematic point when its
+    code object is cached (the benchmark's kernel), and the generic one with
+    FEYNTROP_AMD_SPECIALIZE=0: each printed value and error equals the
+    engine's run of that kernel on the same samples (1e-13)."""
+    import os
+    import feyntrop_amd
+    prob = W.double_box(N=10**6, seed=4)
+    p = engine.prepare(prob)
+    engine.set_specialized(p, True)    # its code object is now in the cache
+    engine.set_specialized(p, False)
+    assert engine.use_cached_specialization(p)
+    for env, spec in ((None, True), ("0", False)):
+        engine.set_specialized(p, spec)
+        ref = p.finalize(engine.integrate(p, prob["N"], prob["seed"]))
+        e = dict(os.environ)
+        e.pop("FEYNTROP_AMD_SPECIALIZE", None)
+        if env is not None:
+            e["FEYNTROP_AMD_SPECIALIZE"] = env
+        out = subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps(prob), capture_output=True, text=True,
+                             timeout=300, env=e)
+        assert out.returncode == 0, out.stderr
+        a = np.array(json.loads(out.stdout)["integral"], dtype=float)
+        np.testing.assert_allclose(a, ref, rtol=1e-13, atol=0, err_msg=f"specialised={spec}")
+
+
 def test_cli_batch_mode_equals_single_runs():
     """{"batch": [...]}: one result per problem, each the single-problem run
     of the binary: the same samples, so IGtr identical and every value and
