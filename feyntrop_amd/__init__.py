@@ -98,6 +98,10 @@ def lib():
     L.ftr_autotune.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.ftr_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
     L.ftr_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.ftr_set_lanes_per_sample.argtypes = [vp, vp, C.c_int]
+    L.ftr_get_lanes_per_sample.argtypes = [vp, C.POINTER(C.c_int)]
+    L.ftr_precompile_variant.argtypes = [C.POINTER(ftr_input), C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                         C.c_int]
     L.ftr_precompile.argtypes = [C.POINTER(ftr_input), C.c_char_p, C.c_int]
     L.ftr_set_jtable_mode.argtypes = [vp, C.c_int]
     L.ftr_batch_graph_replays.argtypes = [vp]
@@ -185,6 +189,17 @@ def precompile_specialized(prob):
     inp = _Input(prob)
     err = C.create_string_buffer(8192)
     rc = lib().ftr_precompile_specialized(C.byref(inp.s), err, 8192)
+    if rc:
+        raise FeyntropError(rc, err.value.decode(errors="replace"))
+
+
+def precompile_variant(prob, table_mode, block, lps=1, specialized=False):
+    """Compile one kernel variant (table placement, block size, lanes per
+    sample; optionally specialised to the kinematic point) into the cache."""
+    inp = _Input(prob)
+    err = C.create_string_buffer(8192)
+    rc = lib().ftr_precompile_variant(C.byref(inp.s), int(table_mode), int(block), int(lps),
+                                      1 if specialized else 0, err, 8192)
     if rc:
         raise FeyntropError(rc, err.value.decode(errors="replace"))
 
@@ -346,6 +361,17 @@ class Engine:
         rc = lib().ftr_set_variant(self.h, p.h, int(table_mode), int(block))
         if rc:
             raise self._err(rc)
+
+    def set_lanes_per_sample(self, p, lps):
+        """1: one sample per lane; 2 (deformed path): a lane pair per sample."""
+        rc = lib().ftr_set_lanes_per_sample(self.h, p.h, int(lps))
+        if rc:
+            raise self._err(rc)
+
+    def lanes_per_sample(self, p):
+        v = C.c_int()
+        lib().ftr_get_lanes_per_sample(p.h, C.byref(v))
+        return v.value
 
     def set_specialized(self, p, on=True):
         """Specialise p's integration kernel to its kinematic point."""

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -29,6 +30,7 @@ using ftrh::Error;
 struct Variant {
   int mode = 0, tmode = 0, block = 256, K = 1;
   int cparams = 0; /* 1: the integration kernel is specialised to the kinematic point */
+  int lps = 1;     /* lanes per sample: 2 on the deformed path splits the Jacobian's rows over a lane pair */
 };
 
 struct Module {
@@ -47,7 +49,8 @@ struct ftr_ctx {
   std::map<std::string, Module> modules;
   std::map<std::string, hipModule_t> raw_modules; /* every loaded code object */
   std::map<std::string, Module> spec_modules;     /* the same modules, keyed by the graph spec alone */
-  std::map<std::string, std::pair<int, int>> tune_memo; /* tuning file -> (table mode, block), -1: none */
+  std::map<std::string, std::array<int, 3>> tune_memo; /* tuning file -> (table mode, block, lanes per
+                                                        * sample), table mode -1: none */
   /* jump polynomials per (S, 2E-1, log2 block, log2 blocks) and the kernel
    * that chains the block start states */
   std::map<std::tuple<uint64_t, uint64_t, int, int>, std::vector<uint64_t>> jump_cache;
@@ -121,11 +124,14 @@ ftr::DevParams make_params(const ftrh::Prep& pr);
 std::string code_object(const std::string& src, std::string* path_out);
 /* load (once per context) and resolve the kernels named `names` */
 hipModule_t load_module_raw(ftr_ctx* c, const std::string& src);
-/* ext (optional): a caller-provided device scratch of ext_bytes instead of
- * the context's growable one; a launch that does not fit is an FTR_ERR_LIMIT
- * error. */
+/* Launch geometry and stream positions of n samples from sample `first`:
+ * blocks of `block` threads, lps threads per sample slot (each slot draws a
+ * contiguous run of S samples); lanes_req sample slots, or 0 to fill the
+ * device.  ext (optional): a caller-provided device scratch of ext_bytes
+ * instead of the context's growable one; a launch that does not fit is an
+ * FTR_ERR_LIMIT error. */
 Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
-                      int nq, hipFunction_t occ_fn, char* ext = nullptr, size_t ext_bytes = 0);
+                      int nq, hipFunction_t occ_fn, char* ext = nullptr, size_t ext_bytes = 0, int lps = 1);
 /* the kernel that writes the block start states (loaded once per context) */
 hipFunction_t block_states_fn(ftr_ctx* c);
 void ensure_scratch(ftr_ctx* c, size_t bytes);
@@ -135,9 +141,11 @@ size_t variant_lds(const Variant& v, int E);
 /* the variant keeps the deformation's second derivatives in LDS: every
  * deformed-path row-table kernel with blocks of at most 256 threads and
  * E <= FTR_DD_LDS_MAX_E (E(E+1)/2 doubles per thread; 110 KiB per block at
- * E = 10).  FEYNTROP_AMD_DDL=0 keeps them in registers instead. */
+ * E = 10), and every two-lanes-per-sample kernel (E(E+1)/2 doubles per lane
+ * pair).  FEYNTROP_AMD_DDL=0 keeps them in registers in one-lane kernels. */
 int dd_lds_override();
 inline bool dd_in_lds(const Variant& v, int E) {
+  if (v.mode == 2 && v.lps == 2) return true;
   if (dd_lds_override() == 0) return false;
   return v.mode == 2 && v.tmode == 3 && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
 }

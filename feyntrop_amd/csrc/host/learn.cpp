@@ -375,6 +375,7 @@ int ftr_learner_create(ftr_ctx* c, ftr_problem* p, uint64_t init_seed, ftr_learn
     l->mlp = load_mlp(c);
     l->var = p->var;
     l->var.block = LEARN_BLOCK;
+    l->var.lps = 1; /* the learned kernels evaluate one sample per lane */
     /* the learned kernels are specialised to the problem's kinematic point
      * (its parameters compile-time constants, as ftr_set_specialized does
      * for the plain kernel): a learner belongs to one point anyway */

@@ -110,6 +110,18 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
       v.block = blk;
       out.push_back(v);
     }
+  /* two lanes per sample: deformed path, rows in HBM, second derivatives
+   * in LDS per lane pair */
+  if (pr.path == 2 && rows_bytes(E) <= mem_limit)
+    for (int blk : {128, 256, 512}) {
+      Variant v;
+      v.K = pr.K;
+      v.mode = pr.path;
+      v.tmode = 3;
+      v.block = blk;
+      v.lps = 2;
+      if (variant_lds(v, E) + 16 <= lds_limit) out.push_back(v);
+    }
   return out;
 }
 
@@ -118,7 +130,7 @@ size_t variant_lds(const Variant& v, int E) {
   t = (t + 15) & ~(size_t)15;
   /* table mode 3: each thread's E slots for the sampler's scatter of X */
   if (v.tmode == 3) t += (size_t)E * v.block * 8;
-  if (dd_in_lds(v, E)) t += (size_t)(E * (E + 1) / 2) * v.block * 8;
+  if (dd_in_lds(v, E)) t += (size_t)(E * (E + 1) / 2) * (v.block / v.lps) * 8;
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
 }
 
@@ -183,7 +195,7 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
   }
   s << "\nstruct " << gname << " {\n  static constexpr int E = " << g.E << ", V = " << g.V << ", NEXT = " << pr.ext.size() << ";\n";
   s << "  static constexpr bool DDL = " << (dd_in_lds(v, g.E) ? "true" : "false") << ";\n";
-  s << "  static constexpr int BLK = " << v.block << ";\n";
+  s << "  static constexpr int BLK = " << v.block << ", LPS = " << v.lps << ";\n";
   /* rank of P's external block (trace and derivatives in low-rank form) */
   s << "  static constexpr int PR = " << p_lowrank(pr, nullptr, nullptr) << ";\n";
   {
@@ -211,8 +223,11 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
      * kernels 0 to -2.7 % (triple ladder); deformed kernels specialised to
      * a kinematic point +0 to +5 % (2-loop tutorial) -- off for those;
      * the batched bucket kernels (many graphs in one kernel, not
-     * instantiated here) +3 % on the 4096-point scan -- off */
-    const bool on = pipe >= 0 ? pipe == 1 : (instantiate && (v.mode != 2 || !v.cparams));
+     * instantiated here) +3 % on the 4096-point scan -- off; two lanes
+     * per sample (kernels for graphs whose one-lane kernel spills) -- off:
+     * the next sample's state adds to the register demand (13-edge
+     * deformed ladder: 144 B of scratch per lane without, 400 B with) */
+    const bool on = pipe >= 0 ? pipe == 1 : (instantiate && v.lps == 1 && (v.mode != 2 || !v.cparams));
     s << "  static constexpr bool PIPE = " << (on ? "true" : "false") << ";\n";
   }
   s << (v.cparams ? "  using PRM = " + gname + "P;\n" : std::string("  using PRM = ftr::DevParams;\n"));
@@ -321,13 +336,15 @@ static bool load_tuned_memo(ftr_ctx* c, const ftrh::Prep& pr, size_t lds_limit, 
   auto it = c->tune_memo.find(path);
   if (it == c->tune_memo.end()) {
     std::ifstream f(path);
-    int tm = -1, blk = 0;
+    int tm = -1, blk = 0, lps = 1;
     if (!(f >> tm >> blk)) tm = -1;
-    it = c->tune_memo.emplace(path, std::make_pair(tm, blk)).first;
+    else if (!(f >> lps)) lps = 1; /* files of two fields: one lane per sample */
+    it = c->tune_memo.emplace(path, std::array<int, 3>{tm, blk, lps}).first;
   }
-  if (it->second.first < 0) return false;
+  const std::array<int, 3>& t = it->second;
+  if (t[0] < 0) return false;
   for (const Variant& v : candidate_variants(pr, lds_limit))
-    if (v.tmode == it->second.first && v.block == it->second.second) {
+    if (v.tmode == t[0] && v.block == t[1] && v.lps == t[2]) {
       *out = v;
       return true;
     }
@@ -549,28 +566,29 @@ hipFunction_t block_states_fn(ftr_ctx* c) {
 }
 
 Geometry setup_stream(ftr_problem* p, uint64_t first, uint64_t n, uint64_t seed, uint64_t lanes_req, int block,
-                      int nq, hipFunction_t occ_fn, char* ext, size_t ext_bytes) {
+                      int nq, hipFunction_t occ_fn, char* ext, size_t ext_bytes, int lps) {
   ftr_ctx* c = p->ctx;
   const int E = p->prep.g.E;
   Geometry G;
+  const int slots = block / lps; /* sample slots per block */
   uint64_t lanes_target = lanes_req;
   if (!lanes_target) {
     int nbpc = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, occ_fn ? occ_fn : p->m.f_int, block, p->lds) != hipSuccess || nbpc < 1)
       nbpc = 1;
-    lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * block;
+    lanes_target = (uint64_t)c->prop.multiProcessorCount * nbpc * slots;
   }
   const uint64_t Smin = 4;
   uint64_t S = (n + lanes_target - 1) / lanes_target;
   if (S < Smin) S = Smin;
   const uint64_t lanes = (n + S - 1) / S;
-  const uint64_t nblocks = (lanes + block - 1) / block;
+  const uint64_t nblocks = (lanes + slots - 1) / slots;
   G.nblocks = nblocks;
   G.S = S;
   G.lanes = lanes;
   const uint64_t per = 2 * (uint64_t)E - 1;
   int lb = 0;
-  while ((1 << lb) < block) lb++;
+  while ((1 << lb) < slots) lb++;
   int ndig = 0;
   while (ndig < 8 && ((uint64_t)1 << (8 * ndig)) < nblocks) ndig++;
   const size_t bytes_states = (size_t)nblocks * 32;
@@ -715,6 +733,31 @@ int ftr_precompile(const ftr_input* in, char* err, int errlen) {
       const Variant v = choose_variant(pr, lim);
       code_object(full_source(pr, v), nullptr);
     }
+  } catch (std::exception& e) {
+    if (err && errlen > 0) {
+      strncpy(err, e.what(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fe ? fe->code : FTR_ERR_INVALID;
+  }
+  return FTR_OK;
+}
+
+int ftr_precompile_variant(const ftr_input* in, int table_mode, int block, int lps, int specialized, char* err,
+                           int errlen) {
+  try {
+    ftrh::Prep pr;
+    prep_from_input(pr, in);
+    bool found = false;
+    for (Variant v : candidate_variants(pr, (size_t)160 * 1024))
+      if (v.tmode == table_mode && v.block == block && v.lps == lps) {
+        v.cparams = specialized ? 1 : 0;
+        code_object(full_source(pr, v), nullptr);
+        found = true;
+        break;
+      }
+    if (!found) throw Error(FTR_ERR_LIMIT, "infeasible (table_mode, block, lanes per sample) for this problem");
   } catch (std::exception& e) {
     if (err && errlen > 0) {
       strncpy(err, e.what(), errlen - 1);
@@ -1025,7 +1068,8 @@ static int integrate_impl(ftr_ctx* c, ftr_problem* p, uint64_t N, uint64_t seed,
       throw Error(FTR_ERR_DEVICE, "the no-allocation scratch of this device is unavailable: " + c->na_err);
     if (N > 0) {
       Geometry G = setup_stream(p, 0, N, seed, opts ? opts->lanes : 0, p->var.block, NQ, nullptr,
-                                noalloc ? (char*)c->na_scratch : nullptr, noalloc ? FTR_NOALLOC_SCRATCH_BYTES : 0);
+                                noalloc ? (char*)c->na_scratch : nullptr, noalloc ? FTR_NOALLOC_SCRATCH_BYTES : 0,
+                                p->var.lps);
       ftr::Launch a;
       fill_tables(a, p);
       a.block_state = G.d_states;
@@ -1137,8 +1181,18 @@ int ftr_replay(ftr_ctx* c, ftr_problem* p, uint64_t n, const double* X_dev, cons
     if (n == 0) return FTR_OK;
     const ftr::DevParams* prm = p->d_prm;
     void* args[] = {(void*)&prm, &n, (void*)&X_dev, (void*)&psixi_dev, &out_dev, &dropped_dev};
-    const uint64_t nb = (n + 255) / 256;
-    HIPCHK(hipModuleLaunchKernel(p->m.f_rep, (unsigned)nb, 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+    /* two lanes per sample: blocks of the variant's size, the second
+     * derivatives of its block's samples in dynamic LDS */
+    const bool pair = p->var.mode == 2 && p->var.lps == 2;
+    const int E = p->prep.g.E;
+    const unsigned threads = pair ? (unsigned)p->var.block : 256u;
+    const uint64_t lanes = pair ? 2 * n : n;
+    const size_t lds = pair ? (size_t)(E * (E + 1) / 2) * (p->var.block / 2) * 8 : 0;
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)p->m.f_rep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const uint64_t nb = (lanes + threads - 1) / threads;
+    HIPCHK(hipModuleLaunchKernel(p->m.f_rep, (unsigned)nb, 1, 1, threads, 1, 1, (unsigned)lds, c->stream, args,
+                                 nullptr));
     HIPCHK(hipStreamSynchronize(c->stream));
   } catch (std::exception& e) {
     const Error* fe = dynamic_cast<const Error*>(&e);
@@ -1346,6 +1400,7 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
       if (Ns[i] == 0) continue;
       Variant v = probs[i]->var;
       v.cparams = 0; /* bucket kernels read the parameters from memory */
+      v.lps = 1;     /* and evaluate one sample per lane */
       members.emplace_back(&probs[i]->prep, v);
       index.push_back(i);
     }
@@ -1586,26 +1641,64 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
   return FTR_OK;
 }
 
+/* Switch p to the feasible variant (table_mode, block, lps), keeping its
+ * specialisation; false when no candidate matches. */
+static bool apply_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block, int lps) {
+  for (Variant v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c)))
+    if (v.tmode == table_mode && v.block == block && v.lps == lps) {
+      v.cparams = p->var.cparams;
+      if (v.tmode == 3) device_rows(c, p);
+      p->m = module_for(c, p->prep, v);
+      p->var = v;
+      p->lds = variant_lds(v, p->prep.g.E);
+      if (p->lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
+      return true;
+    }
+  return false;
+}
+
 int ftr_set_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block) {
   if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
   try {
     HIPCHK(hipSetDevice(c->device));
-    for (Variant v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c)))
-      if (v.tmode == table_mode && v.block == block) {
-        v.cparams = p->var.cparams;
-        if (v.tmode == 3) device_rows(c, p);
-        p->m = module_for(c, p->prep, v);
-        p->var = v;
-        p->lds = variant_lds(v, p->prep.g.E);
-        if (p->lds > 64 * 1024)
-          (void)hipFuncSetAttribute((const void*)p->m.f_int, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
-        return FTR_OK;
-      }
+    if (apply_variant(c, p, table_mode, block, p->var.lps)) return FTR_OK;
   } catch (std::exception& e) {
     const Error* fe = dynamic_cast<const Error*>(&e);
     return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
   }
-  return fail(c, FTR_ERR_LIMIT, "infeasible (table_mode, block) for this problem");
+  return fail(c, FTR_ERR_LIMIT, "infeasible (table_mode, block) for this problem at " + std::to_string(p->var.lps) +
+                                    " lane(s) per sample");
+}
+
+int ftr_set_lanes_per_sample(ftr_ctx* c, ftr_problem* p, int lps) {
+  if (!c || !p || !p->m.f_int) return FTR_ERR_INVALID;
+  if (lps != 1 && lps != 2) return fail(c, FTR_ERR_INVALID, "lanes per sample must be 1 or 2");
+  if (lps == p->var.lps) return FTR_OK;
+  if (lps == 2 && p->prep.path != 2)
+    return fail(c, FTR_ERR_LIMIT, "two lanes per sample apply to the deformed (complex) path only");
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    if (lps == 1) {
+      const Variant d = choose_variant(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c));
+      if (apply_variant(c, p, d.tmode, d.block, 1)) return FTR_OK;
+    } else {
+      /* rows in HBM; the current block size when its LDS fits, else 256,
+       * 128, 512 */
+      for (int blk : {p->var.block, 256, 128, 512})
+        if (apply_variant(c, p, 3, blk, 2)) return FTR_OK;
+    }
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
+  }
+  return fail(c, FTR_ERR_LIMIT, "no feasible variant at " + std::to_string(lps) + " lane(s) per sample");
+}
+
+int ftr_get_lanes_per_sample(const ftr_problem* p, int* lps) {
+  if (!p || !lps) return FTR_ERR_INVALID;
+  *lps = p->var.lps;
+  return FTR_OK;
 }
 
 int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block, double* best_ms) {
@@ -1625,26 +1718,38 @@ int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, 
   /* (table placement, block) with the generic kernel ... */
   if (ftr_set_specialized(c, p, 0) != FTR_OK) return FTR_ERR_DEVICE;
   for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c))) {
-    if (ftr_set_variant(c, p, v.tmode, v.block) != FTR_OK) continue;
+    try {
+      HIPCHK(hipSetDevice(c->device));
+      if (!apply_variant(c, p, v.tmode, v.block, v.lps)) continue;
+    } catch (std::exception&) {
+      continue;
+    }
     const double t = probe();
     if (t < best) {
       best = t;
       bestv = p->var;
     }
   }
-  if (best >= 1e300) {
-    ftr_set_variant(c, p, keep.tmode, keep.block);
-    ftr_set_specialized(c, p, keep.cparams);
-    return fail(c, FTR_ERR_DEVICE, "autotune: no variant ran");
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    if (best >= 1e300) {
+      apply_variant(c, p, keep.tmode, keep.block, keep.lps);
+      ftr_set_specialized(c, p, keep.cparams);
+      return fail(c, FTR_ERR_DEVICE, "autotune: no variant ran");
+    }
+    if (!apply_variant(c, p, bestv.tmode, bestv.block, bestv.lps))
+      return fail(c, FTR_ERR_DEVICE, "autotune: the chosen variant is no longer feasible");
+  } catch (std::exception& e) {
+    const Error* fe = dynamic_cast<const Error*>(&e);
+    return fail(c, fe ? fe->code : FTR_ERR_DEVICE, e.what());
   }
-  int rc = ftr_set_variant(c, p, bestv.tmode, bestv.block);
-  if (rc != FTR_OK) return rc;
   {
-    /* the placement and block are recorded per graph; specialisation to a
-     * kinematic point is not, since it costs a compile per point */
+    /* the placement, block and lanes per sample are recorded per graph;
+     * specialisation to a kinematic point is not, since it costs a compile
+     * per point */
     mkdir(cache_dir().c_str(), 0755);
     std::ofstream f(tune_path(p->prep));
-    if (f) f << bestv.tmode << " " << bestv.block << "\n";
+    if (f) f << bestv.tmode << " " << bestv.block << " " << bestv.lps << "\n";
     c->tune_memo.erase(tune_path(p->prep));
   }
   /* ... then the kernel specialised to this kinematic point */

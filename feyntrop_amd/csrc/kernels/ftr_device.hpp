@@ -137,6 +137,39 @@ __host__ __device__ inline cplx cpow_real(cplx z, double y) {
 }
 __host__ __device__ inline bool cfinite(cplx z) { return __builtin_isfinite(z.re) && __builtin_isfinite(z.im); }
 
+/* Two lanes per sample: lanes 2m and 2m+1 of a wave evaluate one sample.
+ * pair_bcast<SRC> gives every lane of a pair the value held by its lane SRC
+ * (0 or 1): a DPP quad permutation, [0,0,2,2] or [1,1,3,3], on both 32-bit
+ * halves -- two VALU moves, no LDS.  With LPS = 1 it is the identity. */
+template <int LPS, int SRC>
+__host__ __device__ inline double pair_bcast(double v) {
+#ifdef FTR_HOST_BUILD
+  return v;
+#else
+  if constexpr (LPS == 1) {
+    return v;
+  } else {
+    static_assert(LPS == 2 && (SRC == 0 || SRC == 1), "pairs of lanes only");
+    constexpr int ctrl = SRC == 0 ? 0xA0 : 0xF5;
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, ctrl, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), ctrl, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  }
+#endif
+}
+template <int LPS, int SRC>
+__host__ __device__ inline cplx pair_bcast(cplx z) {
+  return cplx{pair_bcast<LPS, SRC>(z.re), pair_bcast<LPS, SRC>(z.im)};
+}
+/* the same with the source lane as a value that folds to a constant once
+ * the caller's loop is unrolled */
+template <int LPS, class T>
+__host__ __device__ inline T pair_from(T v, int src) {
+  if constexpr (LPS == 1) return v;
+  else return src ? pair_bcast<LPS, 1>(v) : pair_bcast<LPS, 0>(v);
+}
+
 /* z^n for an integer n by repeated squaring (n uniform across the wave);
  * the principal branch of pow(z, n) coincides with it for integer n. */
 __host__ __device__ inline cplx cipow(cplx z, int n) {
@@ -451,8 +484,6 @@ __host__ __device__ inline T trace_inv_P(const T (&M)[G::V * (G::V + 1) / 2], co
   return tr;
 }
 
-/* out[k] (complex, imag 0 on the real path).  Returns false when dropped
- * (non-finite R0, feynman_integral.hpp:227-234, :254-261, :329-347). */
 /* log of the projective rescale factor (feynman_integral.hpp:201-207):
  * scale = (psi_tr^(-D/2) (psi_tr/xi_tr)^W)^(1/(D L/2 + W)), from
  * log psi_tr and log G (xi_tr = G psi_tr). */
@@ -462,10 +493,70 @@ __host__ __device__ inline double log_scale(const PT& p, double lpsi, double lG)
   return p.inv_dlw * t;
 }
 
-/* PT: the problem's parameters, ftr::DevParams read from memory or a type
- * with the same members as compile-time constants (kernels specialised to
- * one kinematic point). */
-template <class G, int K, int MODE, class PT = DevParams, bool DDL = false>
+/* Determinant of the deformation Jacobian's complex-symmetric matrix
+ * B = X^-1 + i c H (c = def_ref, H in dd, packed lower, entry stride DS:
+ * 1 in registers, samples per block in LDS), multiplied into jdet: left-looking
+ * (Crout) LDL^T, column j needing rows >= j of the finished columns only,
+ * so a factor row dies as the sweep passes it and only the pivots are kept.
+ *
+ * LPS = 1: one lane holds every row.  LPS = 2: the pair's lane h holds the
+ * rows i = 2r + h (local row r); for column j the lane owning row j forms
+ * w_k = d_k L[j][k] and the pivot d_j, and pair_bcast hands them to its
+ * partner; each lane then updates its own rows below j.  The arithmetic of
+ * every entry, and its order, is that of one lane, so both give the same
+ * bits; the pair holds half of the live factor rows each.  A lane's local
+ * row whose index is not below j in that lane (the pair's boundary row) is
+ * updated too and never read back. */
+template <class G, int LPS, int DS, class PT, class DDT>
+__host__ __device__ inline __attribute__((always_inline)) void crout_ldlt(const PT& p, const double (&Xi)[G::E], DDT dd, cplx& jdet) {
+  constexpr int E = G::E;
+  constexpr int RL = (E + LPS - 1) / LPS;
+#ifdef FTR_HOST_BUILD
+  const int h = 0;
+#else
+  const int h = LPS == 1 ? 0 : (int)(threadIdx.x & (LPS - 1));
+#endif
+  /* packed factor rows: local row r (row r LPS + h) holds its entries
+   * k < (r + 1) LPS - 1 from offset LPS r (r + 1) / 2 - r */
+  cplx L[LPS * RL * (RL + 1) / 2 - RL + 1];
+  cplx d[E];
+#pragma unroll
+  for (int j = 0; j < E; j++) {
+    const int oj = j % LPS, rj = j / LPS; /* owner lane and local row of row j */
+    cplx w[E];
+#pragma unroll
+    for (int k = 0; k < j; k++) w[k] = pair_from<LPS>(d[k] * L[LPS * rj * (rj + 1) / 2 - rj + k], oj);
+    cplx dj = cmk(Xi[j], p.def_ref * dd[FTR_TRI(j, j) * DS]);
+#pragma unroll
+    for (int k = 0; k < j; k++) dj = fms(dj, L[LPS * rj * (rj + 1) / 2 - rj + k], w[k]);
+    dj = pair_from<LPS>(dj, oj);
+    d[j] = dj;
+    jdet = jdet * dj;
+    const cplx inv = crecip(dj);
+#pragma unroll
+    for (int r = 0; r < RL; r++) {
+      if ((r + 1) * LPS - 1 > j) {
+        /* this lane's row i = r LPS + h (with LPS = 1: row r); the pair's
+         * last local row lacks its second row when E is odd */
+        if (r * LPS + LPS - 1 < E || r * LPS + h < E) {
+          cplx v = cmk(0.0, p.def_ref * dd[(FTR_TRI(r * LPS, j) + h * (r * LPS) + h * (h + 1) / 2) * DS]);
+#pragma unroll
+          for (int k = 0; k < j; k++) v = fms(v, L[LPS * r * (r + 1) / 2 - r + k], w[k]);
+          L[LPS * r * (r + 1) / 2 - r + j] = v * inv;
+        }
+      }
+    }
+  }
+}
+
+/* The integrand of one sample: out[k] (complex, imag 0 on the real path).
+ * Returns false when dropped (non-finite R0, feynman_integral.hpp:227-234,
+ * :254-261, :329-347).  PT: the problem's parameters, ftr::DevParams read
+ * from memory or a type with the same members as compile-time constants
+ * (kernels specialised to one kinematic point).  DDL: the deformation's
+ * second derivatives in LDS at dd_slots; LPS: lanes per sample (2: the
+ * lane pair shares the Jacobian's factorisation, crout_ldlt). */
+template <class G, int K, int MODE, class PT = DevParams, bool DDL = false, int LPS = 1>
 __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double lscale,
                                           cplx (&out)[K], double* dd_slots = nullptr) {
   constexpr int E = G::E, V = G::V, NT = V * (V + 1) / 2;
@@ -616,10 +707,14 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     }
     /* first and second derivatives of pphi (laplacian.hpp:102-215) */
     double th[E];
-    /* in registers, or (DDL) in this thread's LDS slots (stride G::BLK) */
+    /* in registers, or (DDL) in the sample's LDS slots (stride: samples per
+     * block, G::BLK / LPS).  With two lanes per sample (LPS = 2, always DDL)
+     * both lanes compute every entry and store the same value to the same
+     * slot, so each lane reads back only what it stored itself. */
+    static_assert(LPS == 1 || DDL, "two lanes per sample keep the second derivatives in LDS");
     double dd_regs[DDL ? 1 : E * (E + 1) / 2];
     double* const dd = DDL ? dd_slots : dd_regs;
-    constexpr int DS = DDL ? G::BLK : 1;
+    constexpr int DS = DDL ? G::BLK / LPS : 1;
 #pragma unroll
     for (int a = 0; a < E; a++) {
       const int ka = G::pos[G::hi[a]], la = G::pos[G::lo[a]];
@@ -647,29 +742,7 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
      * column j needs rows >= j of the finished columns only, so the factor
      * rows die as the sweep advances; only the pivots are kept. */
     cplx jdet = cmk(1.0, 0.0);
-    {
-      cplx L[E * (E + 1) / 2];  /* L[i][k] for k < i; row i dies after column i */
-      cplx d[E];
-#pragma unroll
-      for (int j = 0; j < E; j++) {
-        cplx w[E];              /* w_k = d_k L[j][k] */
-#pragma unroll
-        for (int k = 0; k < j; k++) w[k] = d[k] * L[FTR_TRI(j, k)];
-        cplx dj = cmk(Xi[j], p.def_ref * dd[FTR_TRI(j, j) * DS]);
-#pragma unroll
-        for (int k = 0; k < j; k++) dj = fms(dj, L[FTR_TRI(j, k)], w[k]);
-        d[j] = dj;
-        jdet = jdet * dj;
-        const cplx inv = crecip(dj);
-#pragma unroll
-        for (int i = j + 1; i < E; i++) {
-          cplx v = cmk(0.0, p.def_ref * dd[FTR_TRI(i, j) * DS]);
-#pragma unroll
-          for (int k = 0; k < j; k++) v = fms(v, L[FTR_TRI(i, k)], w[k]);
-          L[FTR_TRI(i, j)] = v * inv;
-        }
-      }
-    }
+    crout_ldlt<G, LPS, DS>(p, Xi, dd, jdet);
     if constexpr (!G::NU1) jdet = xprod * jdet;
     /* deformation: theta = def_ref (d pphi + m^2), z = x e^{i theta} */
     cplx cX[E], cXi[E];
@@ -809,15 +882,16 @@ __device__ inline double* xs_slots(char* lds, int E) {
   return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + threadIdx.x : nullptr;
 }
 /* the deformation's second derivatives in LDS (G::DDL): E(E+1)/2 slots per
- * thread after the X slots, same [slot][thread] layout */
+ * sample after the X slots, [slot][sample] layout (the block's
+ * blockDim.x / G::LPS samples; a lane pair shares its sample's slots) */
 template <class G>
 __device__ inline size_t dd_lds_bytes() {
-  return G::DDL ? (size_t)(G::E * (G::E + 1) / 2) * blockDim.x * 8 : 0;
+  return G::DDL ? (size_t)(G::E * (G::E + 1) / 2) * (blockDim.x / G::LPS) * 8 : 0;
 }
 template <class G, int TMODE>
 __device__ inline double* dd_slots(char* lds) {
   return G::DDL ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(G::E) + xs_lds_bytes<TMODE>(G::E)) +
-                      threadIdx.x
+                      threadIdx.x / G::LPS
                 : nullptr;
 }
 /* start of the block reduction's LDS area */
@@ -887,24 +961,28 @@ __device__ inline void apply_poly_lane(Xoshiro& g, const uint64_t* q) {
 }
 
 /* Lane stream positioning: the block's first state comes from the device
- * (ftrj_block_states); lane t applies, for each base-256 digit d_k of t, the
- * jump polynomial of d_k 256^k lanes (lane_poly[255 k + d_k - 1]): one
- * polynomial for blocks of up to 256 threads. */
+ * (ftrj_block_states); sample slot t of the block (the lane, or with LPS
+ * lanes per sample the lane / LPS) applies, for each base-256 digit d_k of
+ * t, the jump polynomial of d_k 256^k slots (lane_poly[255 k + d_k - 1]):
+ * one polynomial for blocks of up to 256 slots. */
+template <int LPS = 1>
 __device__ inline Xoshiro lane_stream(const Launch& a, uint32_t bid) {
   const uint64_t* bs = a.block_state + 4 * (size_t)bid;
   Xoshiro g{bs[0], bs[1], bs[2], bs[3]};
-  const unsigned t = threadIdx.x;
-  for (int k = 0; (1u << (8 * k)) < blockDim.x; k++) {
+  const unsigned t = threadIdx.x / LPS;
+  for (int k = 0; (1u << (8 * k)) < blockDim.x / LPS; k++) {
     const unsigned d = (t >> (8 * k)) & 255u;
     if (d) apply_poly_lane(g, a.lane_poly + 4 * (255 * k + d - 1));
   }
   return g;
 }
 
-/* Main body: each lane draws its contiguous run of samples from the single
- * stream, evaluates the integrand and accumulates (n, drops, S1, S2) per
- * order; the block writes one row of partial moments
- * [n, drops, (s1re, s2re, s1im, s2im) x K]. */
+/* Main body: each sample slot (a lane, or a pair of lanes with
+ * G::LPS = 2) draws its contiguous run of samples from the single stream,
+ * evaluates the integrand and accumulates (n, drops, S1, S2) per order; the
+ * block writes one row of partial moments [n, drops, (s1re, s2re, s1im,
+ * s2im) x K].  Both lanes of a pair draw the same samples and hold the same
+ * values; the pair's second lane leaves the reduction. */
 template <class G, int K, int MODE, int TMODE>
 __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   extern __shared__ __align__(16) char lds[];
@@ -917,10 +995,11 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   double* red = red_area<G, TMODE>(lds);
   double* const dds = dd_slots<G, TMODE>(lds);
 
-  Xoshiro rng = lane_stream(a, bid);
-  const uint64_t lane = (uint64_t)bid * blockDim.x + threadIdx.x;
+  constexpr int LPS = G::LPS;
+  Xoshiro rng = lane_stream<LPS>(a, bid);
+  const uint64_t lane = (uint64_t)bid * (blockDim.x / LPS) + threadIdx.x / LPS;
   const uint64_t i0 = lane * a.S;
-  /* this lane's sample count (one loop register instead of two bounds) */
+  /* this slot's sample count (one loop register instead of two bounds) */
   const uint64_t n = (i0 < a.N) ? ((a.N - i0 < a.S) ? a.N - i0 : a.S) : 0;
 
   double acc[NQ];
@@ -946,7 +1025,7 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
       sample_point<G, MODE != 2, G::PF>(rng, tv, X, lpsi, lG);
     }
     cplx out[K];
-    if (!integrand<G, K, MODE, PT, G::DDL>(p, X, log_scale(p, lpsi, lG), out, dds)) {
+    if (!integrand<G, K, MODE, PT, G::DDL, LPS>(p, X, log_scale(p, lpsi, lG), out, dds)) {
       acc[1] += 1.0;
       continue;
     }
@@ -959,6 +1038,12 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
         acc[4 + 4 * k] += out[k].im;
         acc[5 + 4 * k] += out[k].im * out[k].im;
       }
+    }
+  }
+  if constexpr (LPS > 1) {
+    if (threadIdx.x % LPS) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) acc[q] = 0.0;
     }
   }
   /* wave shuffle tree, then LDS across waves, fixed order: deterministic */
@@ -976,13 +1061,16 @@ __device__ inline void integrate_body(const Launch& a, uint32_t bid) {
   }
 }
 
-/* Replay: evaluate the integrand on given points (one sample per lane).
- * X: n x E, psixi: n x 2 -> out: n x K x 2, dropped: n. */
+/* Replay: evaluate the integrand on given points (one sample per lane;
+ * with G::LPS = 2 on the deformed path, one per lane pair, the second
+ * derivatives in dynamic LDS as in the integration kernel: the block must
+ * be G::BLK threads).  X: n x E, psixi: n x 2 -> out: n x K x 2, dropped: n. */
 template <class G, int K, int MODE>
 __device__ inline void replay_body(const DevParams* prm, uint64_t n, const double* Xin,
                                    const double* psixi, double* out, uint8_t* dropped) {
   constexpr int E = G::E;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int LPS = MODE == 2 ? G::LPS : 1;
+  const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPS;
   if (i >= n) return;
   double X[E];
 #pragma unroll
@@ -990,7 +1078,15 @@ __device__ inline void replay_body(const DevParams* prm, uint64_t n, const doubl
   cplx r[K];
   const double ps = psixi[2 * i], xs = psixi[2 * i + 1];
   const double ls = prm->inv_dlw * (prm->mhalfD * log(ps) + (prm->W != 0.0 ? prm->W * log(ps / xs) : 0.0));
-  const bool ok = integrand<G, K, MODE>(*prm, X, ls, r);
+  bool ok;
+  if constexpr (LPS == 1) {
+    ok = integrand<G, K, MODE>(*prm, X, ls, r);
+  } else {
+    extern __shared__ __align__(16) char lds[];
+    ok = integrand<G, K, MODE, DevParams, true, LPS>(*prm, X, ls, r,
+                                                      reinterpret_cast<double*>(lds) + threadIdx.x / LPS);
+    if (threadIdx.x % LPS) return;
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) {
     out[(i * K + k) * 2] = ok ? r[k].re : 0.0;
@@ -1117,6 +1213,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
  * gpath[i] = cut order. */
 template <class G, int K, int MODE, int TMODE, bool GRAD>
 __device__ inline void learn_body(const LearnLaunch& L) {
+  static_assert(G::LPS == 1, "learned kernels evaluate one sample per lane");
   extern __shared__ __align__(16) char lds[];
   constexpr int E = G::E;
   constexpr int NQ = 2 + 4 * K;
@@ -1532,7 +1629,7 @@ __device__ inline uint32_t persistent_next(uint32_t* counter, uint32_t nwork) {
   extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_int(ftr::Launch a) {                 \
     ftr::integrate_body<G, K, MODE, TMODE>(a, blockIdx.x);                                        \
   }                                                                                               \
-  extern "C" __global__ void __launch_bounds__(256)                                               \
+  extern "C" __global__ void __launch_bounds__(G::LPS == 2 ? (BLOCK) : 256)                       \
       NAME##_rep(const ftr::DevParams* prm, uint64_t n, const double* X, const double* px,        \
                  double* out, uint8_t* dropped) {                                                 \
     ftr::replay_body<G, K, MODE>(prm, n, X, px, out, dropped);                                    \

@@ -45,6 +45,34 @@ def batches():
     return out
 
 
+def lanes_variants():
+    """(problem, table mode, block, lanes per sample, specialised) of the
+    two-lanes-per-sample kernels the tests and the benchmark run, and every
+    candidate variant of the 13-edge deformed ladder (the autotuner test)."""
+    fx = workloads.FIXTURES
+    small = {"1L_3pt_mink": fx["1L_3pt_mink"](), "2L_4pt_ladder": fx["2L_4pt_ladder"](),
+             "2L_5pt_box_pent": fx["2L_5pt_box_pent"](), "triple_ladder": workloads.triple_ladder()}
+    out = [(prob, 3, 256, 2, False) for prob in small.values()]
+    out.append((workloads.triple_ladder(), 3, 256, 2, True))
+    db = workloads.double_box()
+    out += [(db, 3, blk, 2, False) for blk in (128, 256, 512)] + [(db, 3, 512, 2, True), (db, 3, 256, 2, True)]
+    l4 = workloads.LARGE["ladder4_mink"]()
+    out += [(l4, tm, blk, 1, sp) for tm in range(4) for blk in (256, 512, 1024) for sp in (False, True)]
+    out += [(l4, 3, blk, 2, sp) for blk in (128, 256, 512) for sp in (False, True)]
+    return out
+
+
+def _variant(item):
+    from feyntrop_amd import FeyntropError, precompile_variant
+    prob, tm, blk, lps, sp = item
+    try:
+        precompile_variant(prob, tm, blk, lps, sp)
+    except FeyntropError as e:
+        if e.code != -5:   # infeasible variants are skipped, as the autotuner skips them
+            return str(e)
+    return None
+
+
 def _one(item):
     name, fn, learned = item
     try:
@@ -71,6 +99,12 @@ def main():
                 bad += 1
                 print(f"precompile {name}: {err}", file=sys.stderr)
     print(f"precompiled {len(items) - bad}/{len(items)} problems")
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        for err in ex.map(_variant, lanes_variants()):
+            if err:
+                bad += 1
+                print(f"precompile variant: {err}", file=sys.stderr)
+    print("precompiled the lanes-per-sample and autotuner variants")
     from feyntrop_amd import precompile_batch
     for b in batches():
         try:

@@ -174,16 +174,30 @@ int ftr_precompile_batch(int n, const ftr_input* in, char* err, int errlen);
 int ftr_batch_graph_replays(const ftr_ctx* ctx);
 
 /* Autotuner (no reference counterpart; replaces the reference's fixed OpenMP
- * chunking): times every feasible (J-table placement, block size) kernel
- * variant on n_probe samples, keeps the fastest for the problem and records
- * it in the cache directory, where later ftr_prepare calls pick it up; then
- * times the kernel specialised to the problem's kinematic point
- * (ftr_set_specialized) and keeps it for this problem if it is faster.
- * table_mode: 0 whole J-table in LDS, 1 J/r in LDS + rest from L2/HBM,
- * 2 all from L2/HBM. */
+ * chunking): times every feasible (J-table placement, block size, lanes per
+ * sample) kernel variant on n_probe samples, keeps the fastest for the
+ * problem and records it in the cache directory, where later ftr_prepare
+ * calls pick it up; then times the kernel specialised to the problem's
+ * kinematic point (ftr_set_specialized) and keeps it for this problem if it
+ * is faster.  table_mode: 0 whole J-table in LDS, 1 J/r in LDS + rest from
+ * L2/HBM, 2 all from L2/HBM, 3 rows in HBM.  The chosen lanes per sample:
+ * ftr_get_lanes_per_sample. */
 int ftr_autotune(ftr_ctx* ctx, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block,
                  double* best_kernel_ms);
+/* Select the (table_mode, block) variant at the problem's current lanes per
+ * sample; FTR_ERR_LIMIT when it is infeasible for the problem. */
 int ftr_set_variant(ftr_ctx* ctx, ftr_problem* p, int table_mode, int block);
+/* Lanes per sample (no reference counterpart): 1, one sample per lane, or,
+ * on the deformed (complex) path only, 2: a lane pair evaluates one sample,
+ * splitting the rows of the deformation Jacobian's LDL^T between its lanes
+ * (half the live factor rows per lane) with the second derivatives in LDS
+ * per pair; table mode 3, blocks of 128, 256 or 512 threads.  Same samples
+ * and, per sample, the same values as one lane per sample; ftr_opts.lanes
+ * and ftr_moments.lanes then count sample slots (lane pairs).  lps = 1
+ * returns to the default variant.  FTR_ERR_LIMIT on the real paths or
+ * when no variant fits. */
+int ftr_set_lanes_per_sample(ftr_ctx* ctx, ftr_problem* p, int lps);
+int ftr_get_lanes_per_sample(const ftr_problem* p, int* lps);
 /* Specialise the problem's integration kernel to its kinematic point (no
  * reference counterpart): D, P, m^2, nu, lambda and the exponents become
  * compile-time constants of the generated kernel (one hipRTC compile per
@@ -199,6 +213,12 @@ int ftr_get_specialized(const ftr_problem* p, int* on);
 int ftr_use_cached_specialization(ftr_ctx* ctx, ftr_problem* p);
 /* Compile the specialised kernel of a problem into the cache without a GPU. */
 int ftr_precompile_specialized(const ftr_input* in, char* err, int errlen);
+/* Compile one kernel variant of a problem (as ftr_set_variant and
+ * ftr_set_lanes_per_sample select it; specialized: the kernel specialised
+ * to the kinematic point) into the cache without a GPU, for the LDS size of
+ * gfx950 (160 KiB).  FTR_ERR_LIMIT when the variant is infeasible. */
+int ftr_precompile_variant(const ftr_input* in, int table_mode, int block, int lps, int specialized, char* err,
+                           int errlen);
 int ftr_get_variant(const ftr_problem* p, int* table_mode, int* block);
 
 /* Where ftr_prepare builds the 2^E-entry J-table (tropical_sampling.hpp:
