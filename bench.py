@@ -169,6 +169,35 @@ def run_tuned(eng, name, steps=2):
             "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
 
 
+def run_lanes(eng, name="ladder4_mink", steps=2):
+    """Lanes per sample on the 13-edge deformed ladder (V = 10), whose
+    one-lane kernel spills: one lane and a lane pair per sample, generic and
+    specialised to the kinematic point, then the autotuner's choice for a
+    fresh preparation of the graph."""
+    from feyntrop_amd import workloads as W
+    prob = W.LARGE[name]()
+    N = int(prob["N"])
+    p = eng.prepare(prob)
+    fps = flops_per_sample(p.info)
+    out = {"workload": f"{name}: E={p.E} V={p.V} K={p.K}, deformed path, N={N:.0e} per call"}
+    for lps in (1, 2):
+        eng.set_lanes_per_sample(p, lps)
+        for spec in (False, True):
+            eng.set_specialized(p, spec)
+            eng.integrate(p, N // 10, 1)
+            ms = float(np.mean([eng.integrate(p, N, 100 + i).kernel_ms for i in range(steps)]))
+            tm, blk = eng.variant(p)
+            out[f"lps{lps}_{'specialised' if spec else 'generic'}"] = {
+                "kernel_ms": ms, "table_mode": tm, "block": blk,
+                "roofline_frac": fps * N / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+        eng.set_specialized(p, False)
+    q = eng.prepare(prob)
+    tm, blk, ms = eng.autotune(q, n_probe=2_000_000)
+    out["autotune"] = {"table_mode": tm, "block": blk, "lanes_per_sample": eng.lanes_per_sample(q),
+                       "specialised": eng.specialized(q), "probe_kernel_ms": ms}
+    return out
+
+
 def _rel_err(p, m):
     r = p.finalize(m)
     I0 = complex(r[0, 0, 0], r[0, 1, 0])
@@ -323,6 +352,9 @@ def main():
                     help="after the run, one device-to-device copy of this many MiB (the "
                          "reference kernel for scaling HBM traffic counters)")
     ap.add_argument("--scan-points", type=int, default=4096)
+    ap.add_argument("--lanes-per-sample", type=int, default=0, choices=(0, 1, 2),
+                    help="force one lane or a lane pair per sample (deformed path) for the main workload; "
+                         "0: the engine's choice")
     ap.add_argument("--learned-only", action="store_true",
                     help="profiling run: train the learned proposal on the 3-loop ladder and integrate "
                          "N = 1e8 twice with it (one JSON line of kernel times, not the benchmark line)")
@@ -353,6 +385,8 @@ def main():
     N = int(prob["N"])
     eng = feyntrop_amd.Engine(0)
     p = eng.prepare(prob)
+    if args.lanes_per_sample:
+        eng.set_lanes_per_sample(p, args.lanes_per_sample)
     # kernel choice (preparation, untimed): the generic kernel (the one the
     # CLI runs) or the one specialised to this kinematic point, whichever is
     # faster on a probe; the other one is timed after the main line
@@ -392,7 +426,8 @@ def main():
         "config": {"workload": f"{args.workload}: E={p.E} V={p.V} K={p.K} path="
                                f"{['real', 'real-psi', 'deformed-complex'][p.info.path]}",
                    "N_per_step": N, "lambda": prob["lambda"], "dimension": prob["dimension"],
-                   "kernel": "specialised to the kinematic point" if eng.specialized(p) else "generic"},
+                   "kernel": "specialised to the kinematic point" if eng.specialized(p) else "generic",
+                   "lanes_per_sample": eng.lanes_per_sample(p)},
         "time_to_1e-3_s": t_1e3,
         "rel_err_eps0_at_N": rel_at_N,
         "eps0": [I0.real, I0.imag],
@@ -427,7 +462,8 @@ def main():
                         ("wheel_k4", lambda: run_tuned(eng, "wheel_k4")),
                         ("cubic5", lambda: run_tuned(eng, "cubic5")),
                         ("triple_ladder", lambda: run_tuned(eng, "triple_ladder")),
-                        ("large_graph", lambda: run_large(eng))):
+                        ("large_graph", lambda: run_large(eng)),
+                        ("lanes_per_sample", lambda: run_lanes(eng))):
             try:
                 extras[key] = fn()
             except Exception as e:  # an extra must not take the main line down

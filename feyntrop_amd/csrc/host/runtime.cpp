@@ -61,8 +61,9 @@ static uint64_t fnv1a(const std::string& s) {
   return h;
 }
 
-/* Kernel variant for a problem: path -> MODE, table size -> placement. */
-Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit) {
+/* Kernel variant for a problem at one lane per sample: path -> MODE,
+ * table size -> placement. */
+static Variant choose_one_lane(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit) {
   Variant v;
   v.K = pr.K;
   v.mode = pr.path;
@@ -87,6 +88,37 @@ Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit)
     v.tmode = 3; /* rows in HBM: one contiguous read per cut step */
   } else {
     v.tmode = 2;
+  }
+  return v;
+}
+
+/* Kernel variant for a problem: the one-lane choice, or for deformed graphs
+ * whose one-lane kernel spills (E >= 11: the second derivatives no longer
+ * fit in LDS per lane) a lane pair per sample, measured 13.3 -> 11.2 ms
+ * (generic) and 10.9 -> 8.4 ms (specialised) per 1e7 samples on the
+ * 13-edge ladder. */
+Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit) {
+  const Variant v = choose_one_lane(pr, lds_limit, mem_limit);
+  const int E = pr.g.E;
+  if (pr.path == 2 && E > FTR_DD_LDS_MAX_E && v.tmode == 3) {
+    Variant w = v;
+    w.lps = 2;
+    for (int blk : {256, 128}) {
+      w.block = blk;
+      if (variant_lds(w, E) + 16 <= lds_limit) return w;
+    }
+  }
+  return v;
+}
+
+/* A problem's variant inside a batch: bucket kernels read the parameters
+ * from memory and evaluate one sample per lane (a lane-pair variant runs
+ * with the one-lane block size of its row placement, 256). */
+static Variant batch_member_variant(Variant v) {
+  v.cparams = 0;
+  if (v.lps != 1) {
+    v.lps = 1;
+    v.block = 256;
   }
   return v;
 }
@@ -1357,8 +1389,7 @@ int ftr_precompile_batch(int n, const ftr_input* in, char* err, int errlen) {
       std::vector<int> index;
       for (int i = 0; i < n; i++) {
         if (!ok[i]) continue;
-        Variant v = choose_variant(prs[i], lim);
-        v.cparams = 0;
+        Variant v = batch_member_variant(choose_variant(prs[i], lim));
         members.emplace_back(&prs[i], v);
         index.push_back(i);
       }
@@ -1398,9 +1429,7 @@ int ftr_integrate_batch(ftr_ctx* c, int n, ftr_problem* const* probs, const uint
     std::vector<int> index;
     for (int i = 0; i < n; i++) {
       if (Ns[i] == 0) continue;
-      Variant v = probs[i]->var;
-      v.cparams = 0; /* bucket kernels read the parameters from memory */
-      v.lps = 1;     /* and evaluate one sample per lane */
+      const Variant v = batch_member_variant(probs[i]->var);
       members.emplace_back(&probs[i]->prep, v);
       index.push_back(i);
     }
@@ -1680,7 +1709,7 @@ int ftr_set_lanes_per_sample(ftr_ctx* c, ftr_problem* p, int lps) {
   try {
     HIPCHK(hipSetDevice(c->device));
     if (lps == 1) {
-      const Variant d = choose_variant(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c));
+      const Variant d = choose_one_lane(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c));
       if (apply_variant(c, p, d.tmode, d.block, 1)) return FTR_OK;
     } else {
       /* rows in HBM; the current block size when its LDS fits, else 256,

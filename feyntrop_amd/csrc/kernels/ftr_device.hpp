@@ -744,44 +744,37 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
     cplx jdet = cmk(1.0, 0.0);
     crout_ldlt<G, LPS, DS>(p, Xi, dd, jdet);
     if constexpr (!G::NU1) jdet = xprod * jdet;
-    /* deformation: theta = def_ref (d pphi + m^2), z = x e^{i theta} */
-    cplx cX[E], cXi[E];
-    double xne = 0.0;
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      th[e] = (th[e] + p.m2[e]) * p.def_ref;
-      double sn, cs;
-      sincos_fast(th[e], &sn, &cs);
-      cX[e] = cmk(X[e] * cs, X[e] * sn);
-      cXi[e] = cmk(Xi[e] * cs, -Xi[e] * sn);
-      if constexpr (!G::NU1) xne += p.nu[e] * th[e];
-    }
-    /* complex Laplacian La(1/z) in elimination order with v*'s diagonal
-     * raised by alpha (as for the real one), complex-symmetric LDL^T and
-     * tr(La(1/z)^-1 P) on the external block */
+    /* deformation: theta = def_ref (d pphi + m^2), z = x e^{i theta}; the
+     * complex Laplacian La(1/z) (elimination order, v*'s diagonal raised by
+     * alpha as for the real one), prod z and sum m^2 z are accumulated edge
+     * by edge as each z is formed, so no array of z or 1/z is kept */
     cplx cdet, ctr;
+    cplx cxp = cmk(1.0, 0.0), cdot = cmk(0.0, 0.0);
+    double xne = 0.0;
     {
       cplx CM[NT], cdinv[V];
 #pragma unroll
       for (int q = 0; q < NT; q++) CM[q] = cmk(0.0, 0.0);
 #pragma unroll
       for (int e = 0; e < E; e++) {
+        const double t = (th[e] + p.m2[e]) * p.def_ref;
+        double sn, cs;
+        sincos_fast(t, &sn, &cs);
+        const cplx z = cmk(X[e] * cs, X[e] * sn), zi = cmk(Xi[e] * cs, -Xi[e] * sn);
+        if constexpr (!G::NU1) xne += p.nu[e] * t;
         const int a = G::pos[G::lo[e]], b = G::pos[G::hi[e]];
         const int o = (a > b) ? FTR_TRI(a, b) : FTR_TRI(b, a);
-        CM[FTR_TRI(a, a)] = CM[FTR_TRI(a, a)] + cXi[e];
-        CM[FTR_TRI(b, b)] = CM[FTR_TRI(b, b)] + cXi[e];
-        CM[o] = CM[o] - cXi[e];
+        CM[FTR_TRI(a, a)] = CM[FTR_TRI(a, a)] + zi;
+        CM[FTR_TRI(b, b)] = CM[FTR_TRI(b, b)] + zi;
+        CM[o] = CM[o] - zi;
+        cxp = cxp * z;
+        cdot = cdot + p.m2[e] * z;
       }
+      /* complex-symmetric LDL^T and tr(La(1/z)^-1 P) on the external block */
 #pragma unroll
       for (int i = 0; i < V; i++) CM[FTR_TRI(i, i)].re += (G::ord[i] == vs) ? alpha : 0.0;
       cdet = ldlt_inplace<V>(CM, cdinv, ialpha);
       ctr = trace_inv_P<G, cplx, PT>(CM, cdinv, p);
-    }
-    cplx cxp = cmk(1.0, 0.0), cdot = cmk(0.0, 0.0);
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      cxp = cxp * cX[e];
-      cdot = cdot + p.m2[e] * cX[e];
     }
     const cplx cPsi = cxp * cdet;
     const cplx cPXi = cdot - ctr;

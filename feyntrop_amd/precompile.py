@@ -57,7 +57,7 @@ def lanes_variants():
     db = workloads.double_box()
     out += [(db, 3, blk, 2, False) for blk in (128, 256, 512)] + [(db, 3, 512, 2, True), (db, 3, 256, 2, True)]
     l4 = workloads.LARGE["ladder4_mink"]()
-    out += [(l4, tm, blk, 1, sp) for tm in range(4) for blk in (256, 512, 1024) for sp in (False, True)]
+    out += [(l4, tm, blk, 1, False) for tm in range(4) for blk in (256, 512, 1024)] + [(l4, 3, 256, 1, True)]
     out += [(l4, 3, blk, 2, sp) for blk in (128, 256, 512) for sp in (False, True)]
     return out
 

@@ -52,6 +52,13 @@ def _replay(engine, p, X, px):
     return out.cpu().numpy().reshape(n, p.K, 2), drop.cpu().numpy()
 
 
+def _one(engine, prob):
+    p = engine.prepare(prob)
+    engine.set_lanes_per_sample(p, 1)
+    assert engine.lanes_per_sample(p) == 1
+    return p
+
+
 def _pair(engine, prob):
     p = engine.prepare(prob)
     assert p.info.path == 2
@@ -76,7 +83,7 @@ def test_pair_replay_equals_one_lane(engine, oracle, name):
     and 99th percentile within 1.5x)."""
     print(f"[lanes-replay] {name}", flush=True)
     prob = DEFORMED[name]()
-    p1 = engine.prepare(prob)
+    p1 = _one(engine, prob)
     p2 = _pair(engine, prob)
     op = oracle.problem(prob)
     X, px, _ = op.sample(oracle.seed(17), 20001)
@@ -110,7 +117,7 @@ def test_pair_estimate_equals_oracle_one_thread(engine, oracle, name):
     print(f"[lanes-estimate] {name}", flush=True)
     prob = DEFORMED[name]()
     p = _pair(engine, prob)
-    p1 = engine.prepare(prob)
+    p1 = _one(engine, prob)
     ref, dropped = oracle.problem(prob).estimate(prob["N"], prob["seed"], 1, 1)
     m = engine.integrate(p, prob["N"], prob["seed"])
     m1 = engine.integrate(p1, prob["N"], prob["seed"])
@@ -137,7 +144,7 @@ def test_pair_block_sizes_and_one_lane_agree(engine, name, blocks):
     print(f"[lanes-blocks] {name}", flush=True)
     prob = DEFORMED[name]()
     N = 300_001
-    p1 = engine.prepare(prob)
+    p1 = _one(engine, prob)
     base = engine.integrate(p1, N, 5)
     n0, s10, s20 = base.arrays()
     p2 = _pair(engine, prob)
@@ -178,6 +185,7 @@ def test_lanes_per_sample_api(engine):
         engine.set_lanes_per_sample(p, 2)
     assert e.value.code == -5
     q = engine.prepare(W.double_box(N=1000))
+    assert engine.lanes_per_sample(q) == 1      # E = 7: one lane per sample by default
     default = engine.variant(q)
     with pytest.raises(FeyntropError) as e:
         engine.set_lanes_per_sample(q, 3)
@@ -187,6 +195,9 @@ def test_lanes_per_sample_api(engine):
         engine.set_variant(q, 0, 256)   # the pair kernel reads rows from HBM only
     engine.set_lanes_per_sample(q, 1)
     assert engine.lanes_per_sample(q) == 1 and engine.variant(q) == default
+    # a deformed graph whose one-lane kernel spills (13 edges) defaults to a lane pair
+    r = engine.prepare(DEFORMED["ladder4_mink"]())
+    assert engine.lanes_per_sample(r) == 2 and engine.variant(r)[0] == 3
 
 
 def test_autotune_chooses_lanes_per_sample(engine, tmp_path):
@@ -204,7 +215,7 @@ def test_autotune_chooses_lanes_per_sample(engine, tmp_path):
         if f.endswith(".co"):
             os.symlink(os.path.join(kc, f), tmp_path / f)
     prob = DEFORMED["ladder4_mink"]()
-    p = engine.prepare(prob)
+    p = _one(engine, prob)
     ref = engine.integrate(p, 200_000, 1)
     try:
         feyntrop_amd.lib().ftr_set_cache_dir(str(tmp_path).encode())
