@@ -1120,8 +1120,10 @@ __device__ inline void sample_body(const Launch& a, double* Xout, double* pxout)
  * of the current state are loaded together (one memory latency per step,
  * not one per candidate), and the running sum over them in ascending edge
  * order selects the cut as the tropical sampler does (zero weights of
- * absent edges leave the sum unchanged). */
-template <class G, bool KEEP_LY>
+ * absent edges leave the sum unchanged).  XS (row placement): the radius
+ * of the cut edge is stored to this thread's LDS slot xs[e * G::BLK]
+ * instead of selected into X over all E edges. */
+template <class G, bool KEEP_LY, bool XS>
 __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&X)[G::E], double& lpsi,
                                    double& lG, double& wgt, uint64_t& path, double* ly_out, uint64_t ly_stride,
                                    double* xs) {
@@ -1169,7 +1171,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     const int Lc = (int)(st.mmL >> 1);
     if (MM && !mm) lGv = lX;
     MM = mm;
-    if (xs) {
+    if constexpr (XS) {
       xs[e * G::BLK] = XX;
     } else {
 #pragma unroll
@@ -1190,7 +1192,7 @@ __device__ inline void learn_point(const LearnLaunch& L, Xoshiro& rng, double (&
     }
     g = c;
   }
-  if (xs) {
+  if constexpr (XS) {
 #pragma unroll
     for (int k = 0; k < E; k++) X[k] = xs[k * G::BLK];
   }
@@ -1229,7 +1231,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, GRAD>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
+    learn_point<G, GRAD, TMODE == 3>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
     if (GRAD) L.gpath[i] = path;
     cplx out[K];
     if (!integrand<G, K, MODE, PT, G::DDL>(p, X, log_scale(p, lF, lGv), out, dds)) {
@@ -1536,7 +1538,7 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, false>(L, rng, X, lF, lGv, wgt, path, nullptr, 0, xs_slots<TMODE>(lds, E));
+    learn_point<G, false, TMODE == 3>(L, rng, X, lF, lGv, wgt, path, nullptr, 0, xs_slots<TMODE>(lds, E));
 #pragma unroll
     for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
     wout[i] = wgt;
