@@ -32,14 +32,14 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # HBM traffic of one ftrk_int launch (N = 1e8) from two rocprofv3 --pmc runs of
 # this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, timed launches),
 # each scaled by what the same counter read for a 256 MiB device-to-device
-# copy in the same run (FETCH 131088.25 KiB, WRITE 262144 KiB for 262144 KiB
+# copy in the same run (FETCH 131088.1 KiB, WRITE 262144 KiB for 262144 KiB
 # moved).  Per kernel kind: generic (parameters read from memory) and
 # specialised to the kinematic point; both with the row-table placement.
 # Summary and commands in profiles/ftrk_int_double_box.md.
-_FETCH_SCALE = 262144 / 131088.25
+_FETCH_SCALE = 262144 / 131088.125
 PMC_TRAFFIC_BYTES = {
-    ("double_box", "generic"): (379.5 * _FETCH_SCALE + 24.0) * 1024,
-    ("double_box", "specialised"): (366.0 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "generic"): (459.5 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "specialised"): (370.5 * _FETCH_SCALE + 24.0) * 1024,
 }
 
 
