@@ -18,6 +18,7 @@ same bar, and its warning line naming the reference's number of dropped
 samples."""
 
 import json
+import os
 import subprocess
 
 import numpy as np
@@ -107,6 +108,14 @@ def test_drop_mask_on_replay(engine, oracle, oracle_ld, replay_stats, name):
     print(f"   kept {keep.sum()} median {np.median(d):.2e} well-conditioned {well.mean():.4f} "
           f"frac>1e-12 {np.mean(d > 1e-12):.5f} (reference build {rs['frac_gt_1e-12']:.5f}; "
           f"on the mask {np.mean(d[well] > 1e-12):.5f}) max {d.max():.2e}", flush=True)
+    out_dir = os.environ.get("FEYNTROP_AMD_PARITY_OUT")
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, f"replay_drops__{name}.json"), "w") as f:
+            json.dump({"n": n, "dropped": int(drop.sum()), "kept": int(keep.sum()), "median": float(np.median(d)),
+                       "frac_gt_1e-12": float(np.mean(d > 1e-12)), "max": float(d.max()),
+                       "mask_frac": float(well.mean()), "mask_frac_gt_1e-12": float(np.mean(d[well] > 1e-12)),
+                       "reference_build_frac_gt_1e-12": rs["frac_gt_1e-12"]}, f, indent=1)
     assert np.median(d) <= 1e-14
     assert np.mean(d > 1e-12) <= 2.0 * rs["frac_gt_1e-12"] + 1e-4
     assert np.mean(d[well] > 1e-12) <= 0.005
