@@ -63,6 +63,19 @@ def flops_per_sample(info):
     return f
 
 
+def transcendentals_per_sample(info, prob):
+    """Transcendental evaluations of one sample on the real paths, which the
+    flop count excludes: a log and an exp per radial draw (E - 1), the
+    projective rescale's exp, the logs of psi and pxi for the eps expansion
+    (K > 1), and a log and an exp per non-integer power (-D/2, -W)."""
+    E, K = info.E, info.K
+    t = 2 * (E - 1) + 1 + (2 if K > 1 else 0)
+    for x in (-prob["dimension"] / 2, -info.W):
+        if x != round(x):
+            t += 2
+    return t
+
+
 def cpu_baseline(prob, n_samples, threads):
     """The reference binary (oracle/_ref/feyntrop_ref, built from the reference
     sources with its own flags) on a bounded sample of the workload."""
@@ -163,10 +176,16 @@ def run_tuned(eng, name, steps=2):
         ms.append(eng.integrate(p, prob["N"], 100 + i).kernel_ms)
     wall = time.perf_counter() - t0
     fps = flops_per_sample(p.info)
-    return {"workload": f"{name}: E={p.E} V={p.V} K={p.K}", "N_per_step": prob["N"],
-            "value": prob["N"] * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
-            "variant": {"table_mode": tm, "block": blk},
-            "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+    out = {"workload": f"{name}: E={p.E} V={p.V} K={p.K}", "N_per_step": prob["N"],
+           "value": prob["N"] * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
+           "variant": {"table_mode": tm, "block": blk, "lanes_per_sample": eng.lanes_per_sample(p)},
+           "roofline_frac": fps * prob["N"] / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+    if p.info.path != 2:
+        # the work the flop count leaves out (real paths: logs and exps)
+        t = transcendentals_per_sample(p.info, prob)
+        out["transcendentals_per_sample"] = t
+        out["transcendentals_per_s_kernel"] = t * prob["N"] / (np.mean(ms) * 1e-3)
+    return out
 
 
 def run_lanes(eng, name="ladder4_mink", steps=2):

@@ -234,3 +234,29 @@ def test_autotune_chooses_lanes_per_sample(engine, tmp_path):
     n0, s10, s20 = ref.arrays()
     assert np.array_equal(n, n0)
     assert np.allclose(s1, s10, rtol=1e-11, atol=1e-11 * np.abs(s10).max())
+
+
+def test_cli_runs_pair_kernel_on_spilling_graph(oracle):
+    """The drop-in binary on the 13-edge deformed ladder: its default
+    variant is a lane pair per sample (no tuning record needed), and its
+    printed estimate is the oracle's one-thread estimate of the same
+    samples: values within 1e-6 of their standard error, errors within
+    1e-6 relative (the second moments agree to 1e-7; the binary prints
+    full doubles)."""
+    print("[lanes-cli] ladder4_mink", flush=True)
+    import json
+    import subprocess
+    import feyntrop_amd
+    prob = DEFORMED["ladder4_mink"]()
+    out = subprocess.run([feyntrop_amd.BIN_PATH], input=json.dumps(prob), capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout)
+    op = oracle.problem(prob)
+    ref, dropped = op.estimate(prob["N"], prob["seed"], 1, 1)
+    res = op.finalize(ref)
+    for q in (0, 1):
+        v, dv = d["integral"][0][q]
+        assert abs(v - res[0, q, 0]) <= 1e-9 * abs(res[0, q, 0]) + 1e-6 * res[0, q, 1], (q, v, res[0, q])
+        assert abs(dv - res[0, q, 1]) <= 1e-6 * res[0, q, 1], (q, dv, res[0, q])
+
