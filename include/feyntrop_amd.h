@@ -79,7 +79,7 @@ typedef struct {
   uint64_t dropped;
   double kernel_ms;       /* HIP-event time of the sampling kernel(s) */
   double seconds;         /* wall time of the call */
-  uint64_t lanes;         /* lanes launched */
+  uint64_t lanes;         /* sample slots launched (lanes, or lane pairs) */
   uint64_t samples_per_lane;
 } ftr_moments;
 
@@ -91,7 +91,8 @@ typedef struct {
   int block;              /* 0 = the compiled one, else must equal it (256, 512 or 1024) */
   int table_mode;         /* -1 = the compiled one, else must equal it: 0 whole table in LDS,
                            * 1 w in LDS + rest global, 2 all global, 3 rows in HBM */
-  uint64_t lanes;         /* total lanes; 0 = fill the device */
+  uint64_t lanes;         /* total sample slots (lanes, or lane pairs at two lanes per sample);
+                           * 0 = fill the device */
 } ftr_opts;
 
 int ftr_create(int device, ftr_ctx** ctx);

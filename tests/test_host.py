@@ -156,6 +156,28 @@ def test_cli_singleton_vertex():
     assert r.returncode != 0 and "singleton vertex 1" in r.stderr
 
 
+def test_precompile_variants_without_gpu(tmp_path):
+    """A chosen kernel variant compiles on a host without a GPU: the lane-pair
+    kernel of a deformed graph (generic and specialised) writes code objects;
+    a variant that cannot exist is FTR_ERR_LIMIT (-5): two lanes per sample on
+    the real path, or a lane pair with the table in LDS."""
+    lib = feyntrop_amd.lib()
+    lib.ftr_set_cache_dir(str(tmp_path).encode())
+    try:
+        prob = W.fixture_1L_3pt_mink()
+        prob["masses_sqr"] = [0.3, 0.7, 1.9]
+        feyntrop_amd.precompile_variant(prob, 3, 256, lps=2)
+        feyntrop_amd.precompile_variant(prob, 3, 128, lps=2, specialized=True)
+        cos = [f for f in os.listdir(tmp_path) if f.endswith(".co")]
+        assert len(cos) == 2
+        for tm, blk, lps, q in ((3, 256, 2, W.wheel_k4()), (0, 256, 2, prob), (3, 1024, 2, prob)):
+            with pytest.raises(feyntrop_amd.FeyntropError) as e:
+                feyntrop_amd.precompile_variant(q, tm, blk, lps=lps)
+            assert e.value.code == -5
+    finally:
+        lib.ftr_set_cache_dir(None)
+
+
 def test_precompile_without_gpu(tmp_path):
     """hipRTC specialises a new graph for gfx950 on a host without a GPU."""
     lib = feyntrop_amd.lib()
