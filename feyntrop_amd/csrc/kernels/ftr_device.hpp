@@ -668,17 +668,22 @@ __host__ __device__ inline bool integrand(const PT& p, double (&X)[G::E], double
           for (int q = j + 1; q < i; q++) s -= M[FTR_TRI(i, q)] * Li[FTR_TRI(q, j)];
           Li[FTR_TRI(i, j)] = s;
         }
+      /* Gm[i][j] = sum_{k >= i} L^-1[k][i] (L^-1[k][j] d_k^-1): the
+       * products in parentheses once per (k, j), then one FMA per term */
+      double Wk[NT];
+#pragma unroll
+      for (int k = 0; k < V; k++) {
+        Wk[FTR_TRI(k, k)] = dz[k];
+#pragma unroll
+        for (int j = 0; j < k; j++) Wk[FTR_TRI(k, j)] = Li[FTR_TRI(k, j)] * dz[k];
+      }
 #pragma unroll
       for (int i = 0; i < V; i++)
 #pragma unroll
         for (int j = 0; j <= i; j++) {
-          double s = 0.0;
+          double s = Wk[FTR_TRI(i, j)];
 #pragma unroll
-          for (int k = i; k < V; k++) {
-            const double lki = (k == i) ? 1.0 : Li[FTR_TRI(k, i)];
-            const double lkj = (k == j) ? 1.0 : Li[FTR_TRI(k, j)];
-            s += lki * lkj * dz[k];
-          }
+          for (int k = i + 1; k < V; k++) s = fma(Li[FTR_TRI(k, i)], Wk[FTR_TRI(k, j)], s);
           Gm[FTR_TRI(i, j)] = s;
         }
     }

@@ -164,11 +164,12 @@ FTR_MATH_FN void sincos_fast(double x, double* sp, double* cp) {
   double r;
   int q;
   if (__builtin_fabs(x) <= 0x1p20) {
-    const double k = __builtin_rint(x * 0x1.45f306dc9c883p-1);
+    /* + 0.0 turns a -0 from rint into +0, so that for k = 0 the three
+     * FMAs add -0 to x and return x itself, a zero's sign included */
+    const double k = __builtin_rint(x * 0x1.45f306dc9c883p-1) + 0.0;
     r = __builtin_fma(-k, 0x1.921fb54400000p+0, x);
     r = __builtin_fma(-k, 0x1.0b4611a600000p-34, r);
     r = __builtin_fma(-k, 0x1.3198a2e037073p-69, r);
-    if (k == 0.0) r = x; /* keeps the sign of a zero */
     q = (int)k;
   } else {
     r = reduce_pio2_large(x, &q);
