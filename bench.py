@@ -38,8 +38,8 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 # Summary and commands in profiles/ftrk_int_double_box.md.
 _FETCH_SCALE = 262144 / 131088.125
 PMC_TRAFFIC_BYTES = {
-    ("double_box", "generic"): (459.5 * _FETCH_SCALE + 24.0) * 1024,
-    ("double_box", "specialised"): (370.5 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "generic"): (452.0 * _FETCH_SCALE + 24.0) * 1024,
+    ("double_box", "specialised"): (361.5 * _FETCH_SCALE + 24.0) * 1024,
 }
 
 
