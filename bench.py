@@ -138,10 +138,19 @@ def run_scan(eng, n_points, seed=0):
             "dropped": int(sum(m.dropped for m in outs))}
 
 
+# HBM bytes per sample of the 22-edge ladder's kernel (folded rows, table
+# mode 4; N = 1e8 per launch): rocprofv3 --pmc FETCH_SIZE of this script's
+# --workload ladder7 run, 1.856e8 KiB per launch, scaled by the same counter
+# for a 256 MiB device copy in that run (131085.4 KiB counted); the sampler's
+# row reads are the bound there.  Summary in profiles/ftrk_int_ladder7.md.
+LADDER7_HBM_BYTES_PER_SAMPLE = 1.856e8 * (262144 / 131085.375) * 1024 / 1e8
+PEAK_HBM_BYTES_S = 8.0e12   # MI355X HBM3E
+
+
 def run_large(eng, name="ladder7", steps=2):
     """A 22-edge planar ladder: the 2^22-entry J-table built on the device
     (timed against the host pass), then plain sampling with the tables read
-    from L2/HBM."""
+    from HBM (folded rows, table mode 4)."""
     from feyntrop_amd import workloads as W
     prob = W.LARGE[name]()
     out = {}
@@ -157,9 +166,15 @@ def run_large(eng, name="ladder7", steps=2):
     ms = [eng.integrate(p, N, 100 + i).kernel_ms for i in range(steps)]
     wall = time.perf_counter() - t0
     fps = flops_per_sample(p.info)
+    tm, blk = eng.variant(p)
     out.update({"workload": f"{name}: E={p.E} V={p.V} K={p.K}, 2^{p.E} J-table entries",
                 "value": N * steps / wall, "unit": "samples/s", "kernel_ms_mean": float(np.mean(ms)),
+                "variant": {"table_mode": tm, "block": blk},
                 "roofline_frac": fps * N / (np.mean(ms) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS})
+    if name == "ladder7" and tm == 4:
+        bps = LADDER7_HBM_BYTES_PER_SAMPLE
+        out.update({"bound": "hbm", "hbm_bytes_per_sample_pmc": bps,
+                    "hbm_frac": bps * N / (np.mean(ms) * 1e-3) / PEAK_HBM_BYTES_S})
     return out
 
 

@@ -92,6 +92,7 @@ struct ftr_problem {
   std::shared_ptr<void> d_shared;
   double* d_rows = nullptr;
   ftr::StateRec* d_rec = nullptr;
+  int rows_tm = 0; /* placement whose rows d_rows holds (3 or 4; 0: none) */
   ftr::DevParams* d_prm = nullptr;
   double *d_w = nullptr, *d_jr = nullptr, *d_ir = nullptr;
   uint8_t* d_mml = nullptr;
@@ -147,20 +148,25 @@ int dd_lds_override();
 inline bool dd_in_lds(const Variant& v, int E) {
   if (v.mode == 2 && v.lps == 2) return true;
   if (dd_lds_override() == 0) return false;
-  return v.mode == 2 && v.tmode == 3 && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
+  return v.mode == 2 && v.tmode >= 3 && v.block <= 256 && E <= FTR_DD_LDS_MAX_E;
 }
 int fail(ftr_ctx* c, int code, const std::string& msg);
 
 /* the J-table on the device (jtable_gpu.cpp); fills pr.Jr/r/mm/Lt */
 void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::JtResult& res);
-/* build the row tables of table mode 3 on the device (jtable_gpu.cpp):
- * device_rows allocates and waits; launch_rows fills rows_region_bytes(E)
+/* build the row tables of placement tm (table mode 3 or 4) on the device
+ * (jtable_gpu.cpp): device_rows allocates (unless the rows of that
+ * placement are there) and waits; launch_rows fills rows_region_bytes(tm, E)
  * at d on the context's stream without waiting */
-void device_rows(ftr_ctx* c, ftr_problem* p);
-size_t rows_region_bytes(int E);
-void launch_rows(ftr_ctx* c, ftr_problem* p, char* d);
+void device_rows(ftr_ctx* c, ftr_problem* p, int tm);
+size_t rows_region_bytes(int tm, int E);
+void launch_rows(ftr_ctx* c, ftr_problem* p, int tm, char* d);
 /* bytes of the table-mode-3 tables of a graph with E edges */
 inline size_t rows_bytes(int E) { return ((size_t)1 << E) * ((size_t)8 * E + sizeof(ftr::StateRec)); }
+/* table mode 4: folded rows */
+inline size_t frows_bytes(int E) { return ((size_t)1 << E) * 8 * (size_t)ftr::frow_stride(E); }
+/* device bytes of the row tables of placement tm (3 or 4) */
+inline size_t table_rows_bytes(int tm, int E) { return tm == 4 ? frows_bytes(E) : rows_bytes(E); }
 /* the table pointers of a problem into a launch */
 void fill_tables(ftr::Launch& a, const ftr_problem* p);
 /* smallest E for which ftr_prepare builds the J-table on the device in auto mode */

@@ -139,31 +139,43 @@ void device_jtable(ftr_ctx* c, ftrh::Prep& pr, const ftrh::JtSetup& js, ftrh::Jt
   pr.jtable_on_device = true;
 }
 
-size_t rows_region_bytes(int E) {
+size_t rows_region_bytes(int tm, int E) {
   const size_t NT = (size_t)1 << E;
+  if (tm == 4) return frows_bytes(E);
   return ((NT * E * 8 + 255) & ~(size_t)255) + NT * sizeof(ftr::StateRec);
 }
 
-void launch_rows(ftr_ctx* c, ftr_problem* p, char* d) {
+void launch_rows(ftr_ctx* c, ftr_problem* p, int tm, char* d) {
   const int E = p->prep.g.E;
   const uint64_t NT = (uint64_t)1 << E;
   hipModule_t mod = load_module_raw(c, ftr_embedded_jtable_source);
   hipFunction_t f = nullptr;
-  HIPCHK(hipModuleGetFunction(&f, mod, "ftrj_rows"));
-  p->d_rows = (double*)d;
-  p->d_rec = (ftr::StateRec*)(d + ((NT * E * 8 + 255) & ~(size_t)255));
   uint64_t nt = NT;
   int e = E;
+  p->d_rows = (double*)d;
+  p->rows_tm = tm;
+  if (tm == 4) {
+    HIPCHK(hipModuleGetFunction(&f, mod, "ftrj_frows"));
+    p->d_rec = nullptr;
+    void* args[] = {&nt, &e, &p->d_w, &p->d_ir, &p->d_mml, &p->d_rows};
+    HIPCHK(hipModuleLaunchKernel(f, (unsigned)((NT + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
+    return;
+  }
+  HIPCHK(hipModuleGetFunction(&f, mod, "ftrj_rows"));
+  p->d_rec = (ftr::StateRec*)(d + ((NT * E * 8 + 255) & ~(size_t)255));
   void* args[] = {&nt, &e, &p->d_w, &p->d_jr, &p->d_ir, &p->d_mml, &p->d_rows, &p->d_rec};
   HIPCHK(hipModuleLaunchKernel(f, (unsigned)((NT + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, args, nullptr));
 }
 
-void device_rows(ftr_ctx* c, ftr_problem* p) {
-  if (p->d_rows) return;
+void device_rows(ftr_ctx* c, ftr_problem* p, int tm) {
+  if (p->d_rows && p->rows_tm == tm) return;
   HIPCHK(hipSetDevice(c->device));
   char* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, rows_region_bytes(p->prep.g.E)));
+  HIPCHK(hipMalloc((void**)&d, rows_region_bytes(tm, p->prep.g.E)));
+  /* rows of the other placement are replaced (rows inside a batch's shared
+   * allocation stay with it) */
+  if (p->d_rows_slab) (void)hipFree(p->d_rows_slab);
   p->d_rows_slab = d;
-  launch_rows(c, p, d);
+  launch_rows(c, p, tm, d);
   HIPCHK(hipStreamSynchronize(c->stream));
 }

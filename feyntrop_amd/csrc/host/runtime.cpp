@@ -75,8 +75,9 @@ static Variant choose_one_lane(const ftrh::Prep& pr, size_t lds_limit, size_t me
     /* the row layout (all E weights of a state in flight at once, the cut
      * found by counting) beats the LDS scans: deformed path +5..9 % on 5- to
      * 10-edge graphs, real path +2 % (K4 wheel) and +7 % (14-edge cubic
-     * graph, against w in LDS with 1024-thread blocks) */
-    v.tmode = 3;
+     * graph, against w in LDS with 1024-thread blocks); folded rows for
+     * tables far beyond the L2 */
+    v.tmode = E >= FTR_FROWS_MIN_E ? 4 : 3;
   } else if (full <= cap0) {
     v.tmode = 0;
   } else if (wonly <= lds_limit) {
@@ -84,8 +85,8 @@ static Variant choose_one_lane(const ftrh::Prep& pr, size_t lds_limit, size_t me
      * big one on the real path (the deformed path needs > 128 VGPRs) */
     v.tmode = 1;
     if (pr.path != 2 && wonly + 16 * (2 + 4 * pr.K) * 8 <= lds_limit) v.block = 1024;
-  } else if (rows_bytes(E) <= mem_limit) {
-    v.tmode = 3; /* rows in HBM: one contiguous read per cut step */
+  } else if (frows_bytes(E) <= mem_limit) {
+    v.tmode = 4; /* folded rows in HBM: one contiguous read per cut step */
   } else {
     v.tmode = 2;
   }
@@ -100,7 +101,7 @@ static Variant choose_one_lane(const ftrh::Prep& pr, size_t lds_limit, size_t me
 Variant choose_variant(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit) {
   const Variant v = choose_one_lane(pr, lds_limit, mem_limit);
   const int E = pr.g.E;
-  if (pr.path == 2 && E > FTR_DD_LDS_MAX_E && v.tmode == 3) {
+  if (pr.path == 2 && E > FTR_DD_LDS_MAX_E && v.tmode >= 3) {
     Variant w = v;
     w.lps = 2;
     for (int blk : {256, 128}) {
@@ -123,18 +124,21 @@ static Variant batch_member_variant(Variant v) {
   return v;
 }
 
-/* Feasible (table placement, block size) pairs for the autotuner. */
+/* Feasible (table placement, block size) pairs.  The autotuner's list
+ * (tuning) leaves out the folded rows of tables that fit the L2 (E below
+ * FTR_FROWS_MIN_E - 2); ftr_set_variant accepts them. */
 static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_limit,
-                                               size_t mem_limit = (size_t)128 << 30) {
+                                               size_t mem_limit = (size_t)128 << 30, bool tuning = false) {
   std::vector<Variant> out;
   const int E = pr.g.E;
   const size_t full = ((size_t)25 << E) + 64, wonly = (size_t)8 << E;
-  for (int tm = 0; tm < 4; tm++)
+  for (int tm = 0; tm < 5; tm++)
     for (int blk : {256, 512, 1024}) {
-      const size_t t = tm == 0 ? full : (tm == 1 ? wonly : (tm == 3 ? (size_t)E * blk * 8 : 0));
+      const size_t t = tm == 0 ? full : (tm == 1 ? wonly : (tm >= 3 ? (size_t)E * blk * 8 : 0));
       if (t + (size_t)(blk / 64) * (2 + 4 * pr.K) * 8 + 16 > lds_limit) continue;
       if (tm == 0 && full > 64 * 1024 && blk == 256) continue;
-      if (tm == 3 && rows_bytes(E) > mem_limit) continue;
+      if (tm >= 3 && table_rows_bytes(tm, E) > mem_limit) continue;
+      if (tm == 4 && tuning && E < FTR_FROWS_MIN_E - 2) continue;
       Variant v;
       v.K = pr.K;
       v.mode = pr.path;
@@ -144,24 +148,27 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
     }
   /* two lanes per sample: deformed path, rows in HBM, second derivatives
    * in LDS per lane pair */
-  if (pr.path == 2 && rows_bytes(E) <= mem_limit)
+  for (int tm : {3, 4}) {
+    if (pr.path != 2 || table_rows_bytes(tm, E) > mem_limit || (tm == 4 && tuning && E < FTR_FROWS_MIN_E - 2))
+      continue;
     for (int blk : {128, 256, 512}) {
       Variant v;
       v.K = pr.K;
       v.mode = pr.path;
-      v.tmode = 3;
+      v.tmode = tm;
       v.block = blk;
       v.lps = 2;
       if (variant_lds(v, E) + 16 <= lds_limit) out.push_back(v);
     }
+  }
   return out;
 }
 
 size_t variant_lds(const Variant& v, int E) {
   size_t t = v.tmode == 0 ? ((size_t)25 << E) + 64 : (v.tmode == 1 ? ((size_t)8 << E) : 0);
   t = (t + 15) & ~(size_t)15;
-  /* table mode 3: each thread's E slots for the sampler's scatter of X */
-  if (v.tmode == 3) t += (size_t)E * v.block * 8;
+  /* table modes 3, 4: each thread's E slots for the sampler's scatter of X */
+  if (v.tmode >= 3) t += (size_t)E * v.block * 8;
   if (dd_in_lds(v, E)) t += (size_t)(E * (E + 1) / 2) * (v.block / v.lps) * 8;
   return t + (size_t)(v.block / 64) * (2 + 4 * v.K) * 8;
 }
@@ -241,9 +248,18 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
      * edges' phases (ftr_device.hpp integrand) */
     s << "  static constexpr bool NU1 = " << (nu1 ? "true" : "false") << ";\n";
   }
-  /* in-step row prefetch of the sampler (ftr_device.hpp sample_point):
-   * generic deformed-path kernels */
-  s << "  static constexpr bool PF = " << ((v.mode == 2 && !v.cparams) ? "true" : "false") << ";\n";
+  {
+    /* in-step row request of the sampler (ftr_device.hpp sample_point):
+     * generic deformed-path kernels, lane-pair kernels (13-edge deformed
+     * ladder, specialised: 8.31 -> 7.95 ms per 1e7) and folded rows
+     * (22-edge ladder: 57.9 -> 56.5 ms per 1e8; 19 edges 41.4 -> 40.4) */
+    static const int pf = [] {
+      const char* e = getenv("FEYNTROP_AMD_PF");  /* =0 / =1 override */
+      return e ? atoi(e) : -1;
+    }();
+    const bool on = pf >= 0 ? pf == 1 : (v.tmode == 4 || (v.mode == 2 && (!v.cparams || v.lps == 2)));
+    s << "  static constexpr bool PF = " << (on ? "true" : "false") << ";\n";
+  }
   {
     /* software-pipelined sample loop (ftr_device.hpp integrate_body) */
     static const int pipe = [] {
@@ -896,7 +912,7 @@ static void prepare_device(ftr_ctx* c, ftr_problem* p) {
   bind_slab(p, L, d);
   HIPCHK(hipMemcpyHtoDAsync(d, h.data(), L.total, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (p->var.tmode == 3) device_rows(c, p);
+  if (p->var.tmode >= 3) device_rows(c, p, p->var.tmode);
 }
 
 /* Device phase of many problems at once: one allocation shared by all
@@ -915,9 +931,9 @@ static void prepare_device_batch(ftr_ctx* c, const std::vector<ftr_problem*>& ps
   }
   size_t total = slabs;
   for (size_t i = 0; i < ps.size(); i++)
-    if (ps[i]->var.tmode == 3) {
+    if (ps[i]->var.tmode >= 3) {
       o_rows[i] = total;
-      total += al(rows_region_bytes(ps[i]->prep.g.E));
+      total += al(rows_region_bytes(ps[i]->var.tmode, ps[i]->prep.g.E));
     }
   std::vector<char> h(slabs);
   for (size_t i = 0; i < ps.size(); i++) stage_slab(ps[i], SlabLayout(ps[i]->prep.g.E), h.data() + o_slab[i]);
@@ -932,7 +948,7 @@ static void prepare_device_batch(ftr_ctx* c, const std::vector<ftr_problem*>& ps
   for (size_t i = 0; i < ps.size(); i++) {
     ps[i]->d_shared = own;
     bind_slab(ps[i], SlabLayout(ps[i]->prep.g.E), d + o_slab[i]);
-    if (ps[i]->var.tmode == 3) launch_rows(c, ps[i], d + o_rows[i]);
+    if (ps[i]->var.tmode >= 3) launch_rows(c, ps[i], ps[i]->var.tmode, d + o_rows[i]);
   }
   HIPCHK(hipStreamSynchronize(c->stream));
 }
@@ -1676,8 +1692,9 @@ static bool apply_variant(ftr_ctx* c, ftr_problem* p, int table_mode, int block,
   for (Variant v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c)))
     if (v.tmode == table_mode && v.block == block && v.lps == lps) {
       v.cparams = p->var.cparams;
-      if (v.tmode == 3) device_rows(c, p);
-      p->m = module_for(c, p->prep, v);
+      Module m = module_for(c, p->prep, v);
+      if (v.tmode >= 3) device_rows(c, p, v.tmode);
+      p->m = m;
       p->var = v;
       p->lds = variant_lds(v, p->prep.g.E);
       if (p->lds > 64 * 1024)
@@ -1712,10 +1729,11 @@ int ftr_set_lanes_per_sample(ftr_ctx* c, ftr_problem* p, int lps) {
       const Variant d = choose_one_lane(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c));
       if (apply_variant(c, p, d.tmode, d.block, 1)) return FTR_OK;
     } else {
-      /* rows in HBM; the current block size when its LDS fits, else 256,
-       * 128, 512 */
+      /* rows in HBM (the current row placement, else table mode 3); the
+       * current block size when its LDS fits, else 256, 128, 512 */
+      const int tm = p->var.tmode >= 3 ? p->var.tmode : 3;
       for (int blk : {p->var.block, 256, 128, 512})
-        if (apply_variant(c, p, 3, blk, 2)) return FTR_OK;
+        if (apply_variant(c, p, tm, blk, 2)) return FTR_OK;
     }
   } catch (std::exception& e) {
     const Error* fe = dynamic_cast<const Error*>(&e);
@@ -1746,7 +1764,7 @@ int ftr_autotune(ftr_ctx* c, ftr_problem* p, uint64_t n_probe, int* table_mode, 
   Variant bestv = keep;
   /* (table placement, block) with the generic kernel ... */
   if (ftr_set_specialized(c, p, 0) != FTR_OK) return FTR_ERR_DEVICE;
-  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c))) {
+  for (const Variant& v : candidate_variants(p->prep, c->prop.sharedMemPerBlock, rows_mem_limit(c), true)) {
     try {
       HIPCHK(hipSetDevice(c->device));
       if (!apply_variant(c, p, v.tmode, v.block, v.lps)) continue;

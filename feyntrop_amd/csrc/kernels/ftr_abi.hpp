@@ -50,6 +50,19 @@ struct DevParams {
   double Peta[FTR_MAX_V];
 };
 
+/* Folded rows (table mode 4, for tables far larger than the L2): state g's
+ * row of frow_stride(E) doubles holds the running sums
+ * T_q = sum_{e <= q, e in g} w(g\e) of its cut weights in edge order
+ * (absent edges add 0, so T_{E-1} = J(g) for g != 0), then 1/r(g), then
+ * mm | L<<1 in the low 32 bits of the next double: a cut step reads one row
+ * and no record.  The stride is even, so every row starts on 16 bytes. */
+constexpr int frow_stride(int E) { return (E + 3) & ~1; }
+/* smallest E whose default row placement is table mode 4: on 4-point
+ * ladders the folded rows took 29.9 against 22.6 ms per 1e8 samples at
+ * E = 16 (mode-3 tables of 10 MB), 41.4 against 49.9 at E = 19 (96 MB) and
+ * 57.9 against 83.7 at E = 22 (0.87 GB) */
+#define FTR_FROWS_MIN_E 18
+
 /* Per-state record of the row-table placement (table mode 3). */
 struct alignas(16) StateRec {
   double jr;      /* J(g) */
@@ -65,7 +78,8 @@ struct Launch {
   const double* t_jr;      /* J(g), 2^E */
   const double* t_ir;      /* 1/r(g), 2^E */
   const uint8_t* t_mmL;    /* mm | L<<1, 2^E */
-  const double* t_rows;    /* table mode 3: 2^E x E, row g holds w(g\e) for e in g, 0 else */
+  const double* t_rows;    /* table mode 3: 2^E x E, row g holds w(g\e) for e in g, 0 else;
+                            * table mode 4: 2^E folded rows of frow_stride(E) */
   const StateRec* t_rec;   /* table mode 3: 2^E records */
   const uint64_t* block_state; /* nblocks x 4: stream state at the block's first sample */
   const uint64_t* lane_poly;   /* 255 per base-256 digit of the lane index, x 4 words: entry

@@ -214,15 +214,93 @@ __host__ __device__ inline cplx cpow_mW(const PT& p, cplx z) {
 #define FTR_TRI(i, j) ((i) * ((i) + 1) / 2 + (j))
 
 /* ------------------------------------------------------------- sampler -- */
-/* tropical_sampling.hpp:503-564.  Table access goes through the TableView
- * (LDS or global).  X[e] filled in edge order; returns log psi_tr and log G.
+/* tropical_sampling.hpp:503-564, table mode 4 (folded rows).  X[e] filled
+ * in edge order; returns log psi_tr and log G. */
+template <class G, bool PF, bool UNR, class TV>
+__device__ inline void sample_folded(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& lpsi_tr, double& lG) {
+  constexpr int E = G::E;
+  /* Folded rows: the state's row holds the running sums T_q of its cut
+   * weights, so J(g) = T_{E-1} (the reference's J: the same sum in the
+   * same order) and the cut is the first q with T_q > R = U J(g) -- the
+   * number of q with T_q <= R, since the sums are nondecreasing and rise
+   * only at edges of g; when no sum exceeds R the last edge of g is
+   * taken.  The row of the state a cut reaches also carries the 1/r and
+   * the flags of the cut's own step, so a step makes one read (fewer HBM
+   * bytes than a row and a record when the table is far larger than the
+   * L2).  PF: that whole row is requested in the cut's step, so that the
+   * sums the next step scans are in flight during the radial draw;
+   * otherwise they are read at the start of the next step.  Same cuts,
+   * same arithmetic. */
+  uint32_t g = (1u << E) - 1u;
+  const double* rg = tv.row(g);
+  const uint32_t b0 = tv.mmL(rg);
+  bool MM = b0 & 1;
+  int oL = (int)(b0 >> 1);
+  double lGv = 0.0, lF = 0.0, lX = 0.0, XX = 1.0;
+  double rv[E];
+  if constexpr (PF) {
+#pragma unroll
+    for (int q = 0; q < E; q++) rv[q] = rg[q];
+  }
+  auto step = [&](int j) {
+    if constexpr (!PF) {
+      const double* r = tv.row(g);
+#pragma unroll
+      for (int q = 0; q < E; q++) rv[q] = r[q];
+    }
+    const double R = rng.uniform() * rv[E - 1];
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < E; q++) cnt += (rv[q] <= R) ? 1 : 0;
+    const int e = (cnt < E) ? cnt : 31 - __builtin_clz(g);
+    const uint32_t c = g ^ (1u << e);
+    const double* rc = tv.row(c);
+    const double ir = tv.ir(rc);
+    const uint32_t b = tv.mmL(rc);
+    if constexpr (PF) {
+      if (j < E - 1) {
+#pragma unroll
+        for (int q = 0; q < E; q++) rv[q] = rc[q];
+      }
+    }
+    const bool mm = b & 1;
+    const int L = (int)(b >> 1);
+    g = c;
+    if (MM && !mm) lGv = lX;
+    MM = mm;
+    tv.xs[e * G::BLK] = XX;
+    if (L < oL) {
+      lF += lX;
+      oL = L;
+    }
+    if (j < E - 1) {
+      const double U = rng.uniform();
+      lX += ir * log_pos(1. - U);
+      XX = exp(lX);
+    }
+  };
+  if constexpr (UNR) {
+#pragma unroll
+    for (int j = 0; j < E; j++) step(j);
+  } else {
+    for (int j = 0; j < E; j++) step(j);
+  }
+#pragma unroll
+  for (int k = 0; k < E; k++) X[k] = tv.xs[k * G::BLK];
+  lpsi_tr = lF;
+  lG = lGv;
+}
+
+/* tropical_sampling.hpp:503-564, table modes 0-3.  Table access goes
+ * through the TableView (LDS or global).  X[e] filled in edge order; returns
+ * log psi_tr and log G.
  * JSUM (row placement): R = U J(g) takes J(g) as the last running sum of the
  * state's row instead of from its record -- the reference's J is that sum,
  * in the same order, so the cut is the same; measured faster on the real
  * path (cubic5 -7 % generic, -12 % specialised; K4 wheel -1 %) and slower
  * on the deformed one (double box generic +2 %). */
 template <class G, bool JSUM = false, bool PF = false, bool UNR = false, class TV>
-__device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& lpsi_tr,
+__device__ inline void sample_tables(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& lpsi_tr,
                                     double& lG) {
   /* The radius product x = prod (1-U)^(1/r) is carried as its logarithm, so
    * each radial draw costs one log and one exp (the reference: one pow); the
@@ -394,6 +472,16 @@ __device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E
   }
   lpsi_tr = lF;
   lG = lGv;
+}
+
+/* The sampler: table mode 4 (folded rows) or the others. */
+template <class G, bool JSUM = false, bool PF = false, bool UNR = false, class TV>
+__device__ inline void sample_point(Xoshiro& rng, const TV& tv, double (&X)[G::E], double& lpsi_tr,
+                                    double& lG) {
+  if constexpr (TV::TM == 4)
+    sample_folded<G, PF, UNR>(rng, tv, X, lpsi_tr, lG);
+  else
+    sample_tables<G, JSUM, PF, UNR>(rng, tv, X, lpsi_tr, lG);
 }
 
 /* ----------------------------------------------------------- integrand -- */
@@ -832,7 +920,8 @@ struct ParamsFor<DevParams> {
  * (L2-resident); 2: everything from global; 3: rows -- the E cut weights
  * w(g\e) of a state contiguous in HBM plus a 32-byte record per state, so a
  * cut step reads one row and one record instead of |g| scattered entries
- * (large tables). */
+ * (large tables); 4: folded rows -- running sums and the record's fields in
+ * one row (ftr_abi.hpp frow_stride), for tables far larger than the L2. */
 template <int TMODE>
 struct TableView {
   static constexpr int TM = TMODE;
@@ -859,12 +948,23 @@ struct TableView<3> {
   __device__ inline uint32_t mmL(uint32_t g) const { return rec[g].mmL; }
   __device__ inline const double* row(uint32_t g) const { return rows + (size_t)g * E; }
 };
+template <>
+struct TableView<4> {
+  static constexpr int TM = 4;
+  const double* rows;
+  int E;
+  double* xs; /* this thread's E LDS slots for X */
+  __device__ inline const double* row(uint32_t g) const { return rows + (size_t)g * frow_stride(E); }
+  /* the row's tail: 1/r(g) and the mm | L<<1 flags */
+  __device__ inline double ir(const double* r) const { return r[E]; }
+  __device__ inline uint32_t mmL(const double* r) const { return *reinterpret_cast<const uint32_t*>(r + E + 1); }
+};
 
 template <int TMODE>
 __host__ __device__ constexpr size_t table_lds_bytes(int E) {
   return TMODE == 0 ? ((size_t)25 << E) + 64 : (TMODE == 1 ? ((size_t)8 << E) : 0);
 }
-/* With the tables in HBM (TMODE 3) the LDS holds each thread's E slots of X,
+/* With the tables in HBM (TMODE 3, 4) the LDS holds each thread's E slots of X,
  * where the sampler scatters the radius of the cut edge (one store per cut
  * instead of a select over all E edges).  Per-thread LDS slots are laid out
  * [slot][thread] (slot e of this thread at ptr[e * G::BLK], G::BLK = the
@@ -873,11 +973,11 @@ __host__ __device__ constexpr size_t table_lds_bytes(int E) {
  * every offset is a compile-time constant. */
 template <int TMODE>
 __device__ inline size_t xs_lds_bytes(int E) {
-  return TMODE == 3 ? (size_t)E * blockDim.x * 8 : 0;
+  return TMODE >= 3 ? (size_t)E * blockDim.x * 8 : 0;
 }
 template <int TMODE>
 __device__ inline double* xs_slots(char* lds, int E) {
-  return TMODE == 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + threadIdx.x : nullptr;
+  return TMODE >= 3 ? reinterpret_cast<double*>(lds + table_lds_bytes<TMODE>(E)) + threadIdx.x : nullptr;
 }
 /* the deformation's second derivatives in LDS (G::DDL): E(E+1)/2 slots per
  * sample after the X slots, [slot][sample] layout (the block's
@@ -904,7 +1004,11 @@ template <int TMODE>
 __device__ inline TableView<TMODE> stage_table(const Launch& a, int E, char* lds) {
   TableView<TMODE> tv;
   const uint32_t NT = 1u << E;
-  if constexpr (TMODE == 3) {
+  if constexpr (TMODE == 4) {
+    tv.rows = a.t_rows; tv.E = E;
+    tv.xs = xs_slots<4>(lds, E);
+    return tv;
+  } else if constexpr (TMODE == 3) {
     tv.pw = a.t_w; tv.rows = a.t_rows; tv.rec = a.t_rec; tv.E = E;
     tv.xs = xs_slots<3>(lds, E);
     return tv;
@@ -1222,7 +1326,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   const ParamsFor<PT> ps{};
   const PT& p = ps.get(a);
   /* the learned tables are read from L2/HBM: the LDS holds the X slots
-   * (TMODE 3) and the reduction */
+   * (TMODE 3, 4) and the reduction */
   double* red = red_area<G, TMODE>(lds);
   double* xs = xs_slots<TMODE>(lds, E);
   double* const dds = dd_slots<G, TMODE>(lds);
@@ -1236,7 +1340,7 @@ __device__ inline void learn_body(const LearnLaunch& L) {
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, GRAD, TMODE == 3>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
+    learn_point<G, GRAD, TMODE >= 3>(L, rng, X, lF, lGv, wgt, path, GRAD ? L.gbuf + a.N + i : nullptr, a.N, xs);
     if (GRAD) L.gpath[i] = path;
     cplx out[K];
     if (!integrand<G, K, MODE, PT, G::DDL>(p, X, log_scale(p, lF, lGv), out, dds)) {
@@ -1543,7 +1647,7 @@ __device__ inline void learn_sample_body(const LearnLaunch& L, double* Xout, dou
   for (uint64_t i = i0; i < i1; i++) {
     double X[E], lF, lGv, wgt;
     uint64_t path;
-    learn_point<G, false, TMODE == 3>(L, rng, X, lF, lGv, wgt, path, nullptr, 0, xs_slots<TMODE>(lds, E));
+    learn_point<G, false, TMODE >= 3>(L, rng, X, lF, lGv, wgt, path, nullptr, 0, xs_slots<TMODE>(lds, E));
 #pragma unroll
     for (int e = 0; e < E; e++) Xout[i * E + e] = X[e];
     wout[i] = wgt;

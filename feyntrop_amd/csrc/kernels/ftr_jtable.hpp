@@ -334,6 +334,29 @@ extern "C" __global__ void ftrj_sampler_tables(uint64_t NT, const double* Jr, co
   mml[i] = (uint8_t)((mmv[i] ? 1 : 0) | (Lt[i] << 1));
 }
 
+/* Folded rows (sampler table mode 4; the layout of ftr_abi.hpp's
+ * frow_stride, (E + 3) & ~1 doubles per row): the running sums of w(g\e)
+ * over e = 0..E-1 (absent edges add 0; J(g) itself is summed in this
+ * order, so T_{E-1} = J(g) for g != 0), 1/r(g), mm | L<<1 as the low 32
+ * bits of a double, zero padding. */
+extern "C" __global__ void ftrj_frows(uint64_t NT, int E, const double* w, const double* ir, const uint8_t* mml,
+                                      double* rows) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= NT) return;
+  const int RS = (E + 3) & ~1;
+  double* r = rows + g * RS;
+  double T = 0.0;
+  for (int e = 0; e < E; e++) {
+    T += ((g >> e) & 1) ? w[g ^ ((uint64_t)1 << e)] : 0.0;
+    r[e] = T;
+  }
+  r[E] = ir[g];
+  uint32_t* m = reinterpret_cast<uint32_t*>(r + E + 1);
+  m[0] = mml[g];
+  m[1] = 0;
+  if (RS > E + 2) r[E + 2] = 0.0;
+}
+
 /* Row-table placement (sampler table mode 3): rows[g E + e] = w(g\e) for e in
  * g, else 0; rec[g] = {J(g), 1/r(g), mm | L<<1 (as the low 32 bits of the
  * third double), 0}. */

@@ -47,8 +47,9 @@ def batches():
 
 def lanes_variants():
     """(problem, table mode, block, lanes per sample, specialised) of the
-    two-lanes-per-sample kernels the tests and the benchmark run, and every
-    candidate variant of the 13-edge deformed ladder (the autotuner test)."""
+    two-lanes-per-sample kernels the tests and the benchmark run, the folded
+    row placement of the placement tests, and every candidate variant of the
+    13-edge deformed ladder (the autotuner test)."""
     fx = workloads.FIXTURES
     small = {"1L_3pt_mink": fx["1L_3pt_mink"](), "2L_4pt_ladder": fx["2L_4pt_ladder"](),
              "2L_5pt_box_pent": fx["2L_5pt_box_pent"](), "triple_ladder": workloads.triple_ladder()}
@@ -56,6 +57,8 @@ def lanes_variants():
     out.append((workloads.triple_ladder(), 3, 256, 2, True))
     db = workloads.double_box()
     out += [(db, 3, blk, 2, False) for blk in (128, 256, 512)] + [(db, 3, 512, 2, True), (db, 3, 256, 2, True)]
+    out += [(prob, 4, 256, 1, False) for prob in (workloads.cubic5(), workloads.triple_ladder(), db,
+                                                  workloads.wheel_k4())]
     l4 = workloads.LARGE["ladder4_mink"]()
     out += [(l4, tm, blk, 1, False) for tm in range(4) for blk in (256, 512, 1024)] + [(l4, 3, 256, 1, True)]
     out += [(l4, 3, blk, 2, sp) for blk in (128, 256, 512) for sp in (False, True)]

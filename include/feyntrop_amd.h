@@ -90,7 +90,9 @@ typedef struct {
 typedef struct {
   int block;              /* 0 = the compiled one, else must equal it (256, 512 or 1024) */
   int table_mode;         /* -1 = the compiled one, else must equal it: 0 whole table in LDS,
-                           * 1 w in LDS + rest global, 2 all global, 3 rows in HBM */
+                           * 1 w in LDS + rest global, 2 all global, 3 rows in HBM,
+                           * 4 folded rows in HBM (running sums and the state's record
+                           * in one row: graphs of 18 or more edges) */
   uint64_t lanes;         /* total sample slots (lanes, or lane pairs at two lanes per sample);
                            * 0 = fill the device */
 } ftr_opts;
@@ -181,8 +183,8 @@ int ftr_batch_graph_replays(const ftr_ctx* ctx);
  * calls pick it up; then times the kernel specialised to the problem's
  * kinematic point (ftr_set_specialized) and keeps it for this problem if it
  * is faster.  table_mode: 0 whole J-table in LDS, 1 J/r in LDS + rest from
- * L2/HBM, 2 all from L2/HBM, 3 rows in HBM.  The chosen lanes per sample:
- * ftr_get_lanes_per_sample. */
+ * L2/HBM, 2 all from L2/HBM, 3 rows in HBM, 4 folded rows in HBM (tried
+ * from 16 edges on).  The chosen lanes per sample: ftr_get_lanes_per_sample. */
 int ftr_autotune(ftr_ctx* ctx, ftr_problem* p, uint64_t n_probe, int* table_mode, int* block,
                  double* best_kernel_ms);
 /* Select the (table_mode, block) variant at the problem's current lanes per
@@ -192,7 +194,7 @@ int ftr_set_variant(ftr_ctx* ctx, ftr_problem* p, int table_mode, int block);
  * on the deformed (complex) path only, 2: a lane pair evaluates one sample,
  * splitting the rows of the deformation Jacobian's LDL^T between its lanes
  * (half the live factor rows per lane) with the second derivatives in LDS
- * per pair; table mode 3, blocks of 128, 256 or 512 threads.  Same samples
+ * per pair; table mode 3 or 4, blocks of 128, 256 or 512 threads.  Same samples
  * and, per sample, the same values as one lane per sample; ftr_opts.lanes
  * and ftr_moments.lanes then count sample slots (lane pairs).  lps = 1
  * returns to the default variant.  FTR_ERR_LIMIT on the real paths or

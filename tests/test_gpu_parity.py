@@ -248,12 +248,13 @@ def test_estimate_equals_reference_one_thread(engine, golden_problems, name):
 
 
 @pytest.mark.parametrize("name,tmode", [("triple_ladder", 3), ("cubic5", 3), ("cubic5", 1), ("wheel_k4", 0),
-                                        ("double_box", 0), ("double_box", 2)])
+                                        ("double_box", 0), ("double_box", 2), ("cubic5", 4), ("triple_ladder", 4),
+                                        ("double_box", 4), ("wheel_k4", 4)])
 def test_table_placements_equal_reference_one_thread(engine, golden_problems, name, tmode):
     """Every J-table placement draws the reference's samples: rows of cut
     weights in HBM (3), whole table in LDS (0), w in LDS (1), all from
-    global (2).  The reference's one-thread moments; sampler points equal
-    the golden ones."""
+    global (2), folded rows of running sums (4).  The reference's
+    one-thread moments; sampler points equal the golden ones."""
     print(f"[placement] {name} table_mode {tmode}", flush=True)
     p = engine.prepare(W.all_problems()[name]())
     engine.set_variant(p, tmode, 256)
