@@ -158,9 +158,11 @@ def test_cli_singleton_vertex():
 
 def test_precompile_variants_without_gpu(tmp_path):
     """A chosen kernel variant compiles on a host without a GPU: the lane-pair
-    kernel of a deformed graph (generic and specialised) writes code objects;
-    a variant that cannot exist is FTR_ERR_LIMIT (-5): two lanes per sample on
-    the real path, or a lane pair with the table in LDS."""
+    kernel of a deformed graph (generic and specialised) and the folded-row
+    placement (table mode 4) of a real and a deformed graph write code
+    objects; a variant that cannot exist is FTR_ERR_LIMIT (-5): two lanes per
+    sample on the real path, a lane pair with the table in LDS, or a table
+    mode that does not exist."""
     lib = feyntrop_amd.lib()
     lib.ftr_set_cache_dir(str(tmp_path).encode())
     try:
@@ -168,9 +170,14 @@ def test_precompile_variants_without_gpu(tmp_path):
         prob["masses_sqr"] = [0.3, 0.7, 1.9]
         feyntrop_amd.precompile_variant(prob, 3, 256, lps=2)
         feyntrop_amd.precompile_variant(prob, 3, 128, lps=2, specialized=True)
+        wheel = W.wheel_k4()
+        wheel["masses_sqr"] = [0.1, 1, 2, 3, 4, 5]
+        feyntrop_amd.precompile_variant(wheel, 4, 256)
+        feyntrop_amd.precompile_variant(prob, 4, 256, lps=2)
         cos = [f for f in os.listdir(tmp_path) if f.endswith(".co")]
-        assert len(cos) == 2
-        for tm, blk, lps, q in ((3, 256, 2, W.wheel_k4()), (0, 256, 2, prob), (3, 1024, 2, prob)):
+        assert len(cos) == 4
+        for tm, blk, lps, q in ((3, 256, 2, W.wheel_k4()), (0, 256, 2, prob), (3, 1024, 2, prob),
+                                (5, 256, 1, prob)):
             with pytest.raises(feyntrop_amd.FeyntropError) as e:
                 feyntrop_amd.precompile_variant(q, tm, blk, lps=lps)
             assert e.value.code == -5
