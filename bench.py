@@ -395,7 +395,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md but not built)", file=sys.stderr)
+              "(sharding the samples across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     import feyntrop_amd
