@@ -61,6 +61,10 @@ static uint64_t fnv1a(const std::string& s) {
   return h;
 }
 
+/* smallest E from which deformed-path kernels specialised to a kinematic
+ * point run the pipelined sample loop with the in-step row request */
+constexpr int FTR_SPEC_PIPE_MIN_E = 8;
+
 /* Kernel variant for a problem at one lane per sample: path -> MODE,
  * table size -> placement. */
 static Variant choose_one_lane(const ftrh::Prep& pr, size_t lds_limit, size_t mem_limit) {
@@ -257,7 +261,8 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
       const char* e = getenv("FEYNTROP_AMD_PF");  /* =0 / =1 override */
       return e ? atoi(e) : -1;
     }();
-    const bool on = pf >= 0 ? pf == 1 : (v.tmode == 4 || (v.mode == 2 && (!v.cparams || v.lps == 2)));
+    const bool on =
+        pf >= 0 ? pf == 1 : (v.tmode == 4 || (v.mode == 2 && (!v.cparams || v.lps == 2 || g.E >= FTR_SPEC_PIPE_MIN_E)));
     s << "  static constexpr bool PF = " << (on ? "true" : "false") << ";\n";
   }
   {
@@ -269,13 +274,17 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
     /* measured per 1e8 samples: real path cubic5 -2.2 % (specialised) and
      * -2.6 % (generic), K4 wheel -3.4 % (specialised); generic deformed
      * kernels 0 to -2.7 % (triple ladder); deformed kernels specialised to
-     * a kinematic point +0 to +5 % (2-loop tutorial) -- off for those;
+     * a kinematic point, with the in-step row request: 8 and 10 edges -5 %
+     * (box-pentagon 15.5 -> 14.7 ms, triple ladder 20.0 -> 19.0), 7 edges
+     * +4 % (double box 10.65 -> 11.1), 5 edges +-1 % -- on from 8 edges;
      * the batched bucket kernels (many graphs in one kernel, not
      * instantiated here) +3 % on the 4096-point scan -- off; two lanes
      * per sample (kernels for graphs whose one-lane kernel spills) -- off:
      * the next sample's state adds to the register demand (13-edge
      * deformed ladder: 144 B of scratch per lane without, 400 B with) */
-    const bool on = pipe >= 0 ? pipe == 1 : (instantiate && v.lps == 1 && (v.mode != 2 || !v.cparams));
+    const bool on = pipe >= 0 ? pipe == 1
+                              : (instantiate && v.lps == 1 &&
+                                 (v.mode != 2 || !v.cparams || g.E >= FTR_SPEC_PIPE_MIN_E));
     s << "  static constexpr bool PIPE = " << (on ? "true" : "false") << ";\n";
   }
   s << (v.cparams ? "  using PRM = " + gname + "P;\n" : std::string("  using PRM = ftr::DevParams;\n"));
