@@ -61,9 +61,14 @@ static uint64_t fnv1a(const std::string& s) {
   return h;
 }
 
-/* smallest E from which deformed-path kernels specialised to a kinematic
- * point run the pipelined sample loop with the in-step row request */
+/* deformed-path kernels specialised to a kinematic point run the
+ * pipelined sample loop with the in-step row request for graphs of
+ * FTR_SPEC_PIPE_MIN_E to FTR_DD_LDS_MAX_E edges (measured there; larger
+ * one-lane kernels spill already) */
 constexpr int FTR_SPEC_PIPE_MIN_E = 8;
+static bool spec_pipe(const Variant& v, int E) {
+  return v.mode == 2 && v.cparams && v.lps == 1 && E >= FTR_SPEC_PIPE_MIN_E && E <= FTR_DD_LDS_MAX_E;
+}
 
 /* Kernel variant for a problem at one lane per sample: path -> MODE,
  * table size -> placement. */
@@ -262,7 +267,7 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
       return e ? atoi(e) : -1;
     }();
     const bool on =
-        pf >= 0 ? pf == 1 : (v.tmode == 4 || (v.mode == 2 && (!v.cparams || v.lps == 2 || g.E >= FTR_SPEC_PIPE_MIN_E)));
+        pf >= 0 ? pf == 1 : (v.tmode == 4 || (v.mode == 2 && (!v.cparams || v.lps == 2)) || spec_pipe(v, g.E));
     s << "  static constexpr bool PF = " << (on ? "true" : "false") << ";\n";
   }
   {
@@ -276,15 +281,15 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
      * kernels 0 to -2.7 % (triple ladder); deformed kernels specialised to
      * a kinematic point, with the in-step row request: 8 and 10 edges -5 %
      * (box-pentagon 15.5 -> 14.7 ms, triple ladder 20.0 -> 19.0), 7 edges
-     * +4 % (double box 10.65 -> 11.1), 5 edges +-1 % -- on from 8 edges;
+     * +4 % (double box 10.65 -> 11.1), 5 edges +-1 % -- on for 8 to 10
+     * edges (spec_pipe);
      * the batched bucket kernels (many graphs in one kernel, not
      * instantiated here) +3 % on the 4096-point scan -- off; two lanes
      * per sample (kernels for graphs whose one-lane kernel spills) -- off:
      * the next sample's state adds to the register demand (13-edge
      * deformed ladder: 144 B of scratch per lane without, 400 B with) */
     const bool on = pipe >= 0 ? pipe == 1
-                              : (instantiate && v.lps == 1 &&
-                                 (v.mode != 2 || !v.cparams || g.E >= FTR_SPEC_PIPE_MIN_E));
+                              : (instantiate && v.lps == 1 && (v.mode != 2 || !v.cparams || spec_pipe(v, g.E)));
     s << "  static constexpr bool PIPE = " << (on ? "true" : "false") << ";\n";
   }
   s << (v.cparams ? "  using PRM = " + gname + "P;\n" : std::string("  using PRM = ftr::DevParams;\n"));
