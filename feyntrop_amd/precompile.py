@@ -17,6 +17,7 @@ def problems():
     probs["ladder7"] = workloads.LARGE["ladder7"]   # the large-graph bench line
     probs["ladder5"] = workloads.LARGE["ladder5"]   # learned proposal on row tables (tests)
     probs["ladder4_mink"] = workloads.LARGE["ladder4_mink"]   # large deformed graph (tests)
+    probs["ladder6"] = lambda: workloads.ladder_4pt(6)   # folded rows against row tables (tests)
     for k, v in workloads.scan_problems(host_path).items():
         probs[k] = v
     return probs
@@ -59,6 +60,7 @@ def lanes_variants():
     out += [(db, 3, blk, 2, False) for blk in (128, 256, 512)] + [(db, 3, 512, 2, True), (db, 3, 256, 2, True)]
     out += [(prob, 4, 256, 1, False) for prob in (workloads.cubic5(), workloads.triple_ladder(), db,
                                                   workloads.wheel_k4())]
+    out.append((workloads.ladder_4pt(6), 3, 256, 1, False))
     l4 = workloads.LARGE["ladder4_mink"]()
     out += [(l4, tm, blk, 1, False) for tm in range(4) for blk in (256, 512, 1024)] + [(l4, 3, 256, 1, True)]
     out += [(l4, 3, blk, 2, sp) for blk in (128, 256, 512) for sp in (False, True)]
