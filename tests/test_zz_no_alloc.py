@@ -7,8 +7,9 @@ per GPU and sums the block partials on the device in block order, the order
 of ftr_integrate's host sum.  Same kernels and launch geometry, so the
 moments must be equal bit for bit.  Checked on two calls in a row, the
 second with a smaller N (the array keeps the first call's contents), on the
-deformed and the real path, and after a call of the allocating entry point;
-a launch too large for the array is rejected with FTR_ERR_LIMIT."""
+deformed and the real path (the 19-edge ladder with its J-table as folded
+rows too), and after a call of the allocating entry point; a launch too
+large for the array is rejected with FTR_ERR_LIMIT."""
 
 import ctypes as C
 
@@ -37,13 +38,17 @@ def _same(a, b):
             and np.array_equal(s2a, s2b) and a.lanes == b.lanes and a.samples_per_lane == b.samples_per_lane)
 
 
+PROBLEMS = dict(W.all_problems())
+PROBLEMS["ladder6"] = lambda: W.ladder_4pt(6)
+
+
 @pytest.mark.parametrize("name,N1,N2", [("double_box", 3_000_000, 200_000), ("wheel_k4", 5_000_000, 77_777),
-                                        ("drop_double_box_D5.5", 400_000, 65_000)])
+                                        ("drop_double_box_D5.5", 400_000, 65_000), ("ladder6", 2_000_000, 90_001)])
 def test_noalloc_equals_integrate(engine, name, N1, N2):
     import feyntrop_amd
     lib = feyntrop_amd.lib()
     print(f"[noalloc] {name} N {N1} then {N2}", flush=True)
-    p = engine.prepare(W.all_problems()[name]())
+    p = engine.prepare(PROBLEMS[name]())
     for N, seed in ((N1, 5), (N2, 6)):
         rc, m0 = _call(lib, "ftr_integrate_noalloc", engine, p, N, seed)
         assert rc == 0, lib.ftr_last_error(engine.h)
