@@ -134,8 +134,8 @@ static Variant batch_member_variant(Variant v) {
 }
 
 /* Feasible (table placement, block size) pairs.  The autotuner's list
- * (tuning) leaves out the folded rows of tables that fit the L2 (E below
- * FTR_FROWS_MIN_E - 2); ftr_set_variant accepts them. */
+ * (tuning) leaves out the folded rows below FTR_FROWS_MIN_E edges, where
+ * they lose; ftr_set_variant accepts them. */
 static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_limit,
                                                size_t mem_limit = (size_t)128 << 30, bool tuning = false) {
   std::vector<Variant> out;
@@ -147,7 +147,7 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
       if (t + (size_t)(blk / 64) * (2 + 4 * pr.K) * 8 + 16 > lds_limit) continue;
       if (tm == 0 && full > 64 * 1024 && blk == 256) continue;
       if (tm >= 3 && table_rows_bytes(tm, E) > mem_limit) continue;
-      if (tm == 4 && tuning && E < FTR_FROWS_MIN_E - 2) continue;
+      if (tm == 4 && tuning && E < FTR_FROWS_MIN_E) continue;
       Variant v;
       v.K = pr.K;
       v.mode = pr.path;
@@ -158,7 +158,7 @@ static std::vector<Variant> candidate_variants(const ftrh::Prep& pr, size_t lds_
   /* two lanes per sample: deformed path, rows in HBM, second derivatives
    * in LDS per lane pair */
   for (int tm : {3, 4}) {
-    if (pr.path != 2 || table_rows_bytes(tm, E) > mem_limit || (tm == 4 && tuning && E < FTR_FROWS_MIN_E - 2))
+    if (pr.path != 2 || table_rows_bytes(tm, E) > mem_limit || (tm == 4 && tuning && E < FTR_FROWS_MIN_E))
       continue;
     for (int blk : {128, 256, 512}) {
       Variant v;

@@ -57,11 +57,14 @@ struct DevParams {
  * mm | L<<1 in the low 32 bits of the next double: a cut step reads one row
  * and no record.  The stride is even, so every row starts on 16 bytes. */
 constexpr int frow_stride(int E) { return (E + 3) & ~1; }
-/* smallest E whose default row placement is table mode 4: on 4-point
- * ladders the folded rows took 29.9 against 22.6 ms per 1e8 samples at
- * E = 16 (mode-3 tables of 10 MB), 41.4 against 49.9 at E = 19 (96 MB) and
- * 57.9 against 83.7 at E = 22 (0.87 GB) */
-#define FTR_FROWS_MIN_E 18
+/* smallest E whose default row placement is table mode 4.  Generic
+ * kernels, ms per 1e8 samples, mode 3 against 4: 14-edge cubic graph 37.4
+ * against 50.5; 4-point ladders (and diagonals) of 16, 17, 18 edges 24.4
+ * against 20.9, 33.7 against 25.4, 40.0 against 28.7; 22 edges 83.7
+ * against 56.4 (mode-3 tables of 10 MB at E = 16, 0.87 GB at 22).  The
+ * kernel specialised to a kinematic point loses with mode 4 at 16 edges
+ * (22.6 against 29.9 ms) and ties from 19; the autotuner weighs both. */
+#define FTR_FROWS_MIN_E 16
 
 /* Per-state record of the row-table placement (table mode 3). */
 struct alignas(16) StateRec {

@@ -92,7 +92,7 @@ typedef struct {
   int table_mode;         /* -1 = the compiled one, else must equal it: 0 whole table in LDS,
                            * 1 w in LDS + rest global, 2 all global, 3 rows in HBM,
                            * 4 folded rows in HBM (running sums and the state's record
-                           * in one row: graphs of 18 or more edges) */
+                           * in one row: graphs of 16 or more edges) */
   uint64_t lanes;         /* total sample slots (lanes, or lane pairs at two lanes per sample);
                            * 0 = fill the device */
 } ftr_opts;

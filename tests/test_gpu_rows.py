@@ -1,4 +1,4 @@
-"""Folded rows (table mode 4, the default placement from 18 edges) on a
+"""Folded rows (table mode 4, the default placement from 16 edges) on a
 graph whose tables outgrow the caches: the 6-rung planar ladder (E = 19,
 2^19 states).
 
