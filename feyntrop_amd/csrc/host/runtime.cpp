@@ -299,6 +299,12 @@ static std::string spec_source(const ftrh::Prep& pr, const Variant& v, bool lear
   arr(s, "pos", pos);
   s << "};\n";
   if (!instantiate) return s.str();
+  /* folded rows: at most one wave per SIMD (the 16-edge ladder's
+   * specialised kernel, which fits two, ran 30.0 ms per 1e8 against 21.7
+   * at one; the 22-edge one holds one anyway; mode-3 kernels that fit two
+   * lose at one: cubic5 36.6 -> 42.4 ms per 3e8, K4 wheel +10 %) */
+  if (v.tmode == 4)
+    s << "#undef FTR_INT_ATTR\n#define FTR_INT_ATTR __attribute__((amdgpu_waves_per_eu(1, 1)))\n";
   s << "FTR_INSTANTIATE(ftrk, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";
   if (learned)
     s << "FTR_INSTANTIATE_LEARNED(ftrl, FG, " << v.K << ", " << v.mode << ", " << v.tmode << ", " << v.block << ")\n";

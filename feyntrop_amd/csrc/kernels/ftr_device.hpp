@@ -1729,8 +1729,11 @@ __device__ inline uint32_t persistent_next(uint32_t* counter, uint32_t nwork) {
 /* Instantiation macro used by the generated kernel units:
  *   FTR_INSTANTIATE(name, Graph, K, MODE, TMODE, BLOCK)
  * defines extern "C" kernels  name_int, name_rep, name_smp. */
+#ifndef FTR_INT_ATTR
+#define FTR_INT_ATTR /* extra attributes of the integration kernel (compile option) */
+#endif
 #define FTR_INSTANTIATE(NAME, G, K, MODE, TMODE, BLOCK)                                           \
-  extern "C" __global__ void __launch_bounds__(BLOCK) NAME##_int(ftr::Launch a) {                 \
+  extern "C" __global__ void __launch_bounds__(BLOCK) FTR_INT_ATTR NAME##_int(ftr::Launch a) {    \
     ftr::integrate_body<G, K, MODE, TMODE>(a, blockIdx.x);                                        \
   }                                                                                               \
   extern "C" __global__ void __launch_bounds__(G::LPS == 2 ? (BLOCK) : 256)                       \
